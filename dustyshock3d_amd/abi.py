@@ -1,0 +1,179 @@
+"""ctypes mirror of include/dsh.h and the loader of the in-tree libdsh.so.
+
+The library is the product path: nothing here falls back to another implementation.
+If libdsh.so is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdsh.so")
+
+MAX_DUST = 2
+MAX_SPECIES = 1 + MAX_DUST
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_NOMEM = -3
+ERR_STATE = -4
+ERR_GRID = -5
+ERR_NONFINITE = -6
+ERR_IO = -7
+ERR_BUDGET = -8
+ERR_NOTFOUND = -9
+ERR_UNSUPPORTED = -10
+
+DT_FIXED, DT_CFL_ONCE, DT_CFL = 0, 1, 2
+FP64, MIXED = 0, 1
+
+_D2 = C.c_double * 2
+
+
+class Params(C.Structure):
+    """dsh_params: Init.txt lines 2-18 (kernel.cu:1705-1722) followed by extensions."""
+
+    _fields_ = [
+        ("x_min", C.c_double), ("x_max", C.c_double),
+        ("y_min", C.c_double), ("y_max", C.c_double),
+        ("z_min", C.c_double), ("z_max", C.c_double),
+        ("eos_type", C.c_int), ("max_particles", C.c_int),
+        ("particles_per_length", C.c_double), ("h", C.c_double),
+        ("avg_cell_coeff", C.c_double), ("tau", C.c_double),
+        ("t_end", C.c_double), ("t_out", C.c_double),
+        ("alpha", C.c_double), ("beta", C.c_double), ("eps", C.c_double),
+        ("periodic_y", C.c_int), ("periodic_z", C.c_int),
+        ("gamma", C.c_double), ("cs", C.c_double),
+        ("h_mode", C.c_int), ("eta", C.c_double),
+        ("n_dust", C.c_int), ("drag_mode", C.c_int),
+        ("K", _D2), ("R_dust", _D2), ("mass_ratio", _D2), ("eps_fallback", _D2),
+        ("tstop_fallback", C.c_double), ("eps2_uses_count1", C.c_int),
+        ("dt_mode", C.c_int), ("cfl", C.c_double), ("max_steps", C.c_int),
+        ("cell_size", C.c_double), ("cell_slack", C.c_double),
+        ("avg_factor", C.c_int), ("precision", C.c_int),
+    ]
+
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+
+
+class SoaView(C.Structure):
+    """dsh_soa_view: the data.h structure-of-arrays layout (data.h:19-20,49-55)."""
+
+    _fields_ = [
+        ("x", _DP), ("y", _DP), ("z", _DP),
+        ("vx", _DP), ("vy", _DP), ("vz", _DP),
+        ("ax", _DP), ("ay", _DP), ("az", _DP),
+        ("m", _DP), ("rho", _DP), ("p", _DP), ("e", _DP), ("h", _DP),
+        ("t_stop", _DP), ("edot", _DP),
+        ("ind", _IP), ("nn", _IP), ("cell", _IP),
+    ]
+
+
+class TubeSpec(C.Structure):
+    """dsh_tube_spec: one isothermal dusty shock tube."""
+
+    _fields_ = [
+        ("x_min", C.c_double), ("x_max", C.c_double),
+        ("y_width", C.c_double), ("z_width", C.c_double),
+        ("dx_left", C.c_double), ("rho_left", C.c_double),
+        ("dust_to_gas", C.c_double), ("K", C.c_double),
+        ("jitter", C.c_double), ("frozen_h", C.c_double),
+        ("t_end", C.c_double), ("cfl", C.c_double),
+        ("dt_mode", C.c_int), ("max_steps", C.c_int), ("precision", C.c_int),
+        ("tube_index", C.c_int), ("seed", C.c_uint64),
+    ]
+
+
+DOUBLE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "e", "h",
+                 "t_stop", "edot")
+INT_FIELDS = ("ind", "nn", "cell")
+ALL_FIELDS = DOUBLE_FIELDS + INT_FIELDS
+
+
+def empty_arrays(n: int, fields=ALL_FIELDS) -> dict:
+    out = {}
+    for f in fields:
+        out[f] = np.zeros(n, dtype=np.int32 if f in INT_FIELDS else np.float64)
+    return out
+
+
+def make_view(arrays: dict) -> SoaView:
+    """Build a view over numpy arrays (the arrays must stay alive while the view is used)."""
+    v = SoaView()
+    for f, a in arrays.items():
+        if a is None:
+            continue
+        if f in INT_FIELDS:
+            assert a.dtype == np.int32 and a.flags.c_contiguous, f
+            setattr(v, f, a.ctypes.data_as(_IP))
+        else:
+            assert a.dtype == np.float64 and a.flags.c_contiguous, f
+            setattr(v, f, a.ctypes.data_as(_DP))
+    return v
+
+
+def spec_dict(spec: TubeSpec) -> dict:
+    return {k: getattr(spec, k) for k, _ in TubeSpec._fields_}
+
+
+def params_dict(p: Params) -> dict:
+    out = {}
+    for k, _ in Params._fields_:
+        v = getattr(p, k)
+        out[k] = list(v) if isinstance(v, C.Array) else v
+    return out
+
+
+_LIB = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "dsh_default_params": (C.c_int, [P(Params)]),
+        "dsh_tube_params": (C.c_int, [P(TubeSpec), P(Params)]),
+        "dsh_tube_count": (C.c_int, [P(TubeSpec), _IP]),
+        "dsh_tube_ic": (C.c_int, [P(TubeSpec), C.c_int, P(SoaView)]),
+        "dsh_read_init": (C.c_int, [C.c_char_p, P(Params)]),
+        "dsh_create": (C.c_int, [P(Params), P(vp)]),
+        "dsh_upload_species": (C.c_int, [vp, C.c_int, P(SoaView), C.c_int]),
+        "dsh_step": (C.c_int, [vp, C.c_int, P(C.c_double)]),
+        "dsh_download_species": (C.c_int, [vp, C.c_int, P(SoaView)]),
+        "dsh_conservation": (C.c_int, [vp, _DP]),
+        "dsh_status": (C.c_int, [vp, _DP, _DP, _IP, _IP]),
+        "dsh_grid": (C.c_int, [vp, _DP, _IP]),
+        "dsh_set_stats": (C.c_int, [vp, C.c_int]),
+        "dsh_neighbour_stats": (C.c_int, [vp, C.c_int, _IP, P(C.c_uint64)]),
+        "dsh_set_timing": (C.c_int, [vp, C.c_int]),
+        "dsh_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),
+        "dsh_last_error": (C.c_char_p, [vp]),
+        "dsh_destroy": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """Load the in-tree libdsh.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _LIB = _declare(C.CDLL(LIB_PATH))
+    return _LIB
+
+
+class DshError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dsh error {code}: {msg}")
+        self.code = code

@@ -1,0 +1,543 @@
+// dsh_engine.cpp - host side of the batch engine: grids, layout, upload/download,
+// the per-step launch sequence (optionally captured once in a hipGraph), and timing.
+#include "dsh_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace dsh {
+
+void build_grid(const dsh_params& P, double hmin, TubeConst& T) {
+    const double lo[3] = {P.x_min, P.y_min, P.z_min};
+    const double up[3] = {P.x_max, P.y_max, P.z_max};
+    T.per[0] = 0;
+    T.per[1] = P.periodic_y ? 1 : 0;
+    T.per[2] = P.periodic_z ? 1 : 0;
+    T.f = P.avg_factor < 1 ? 1 : P.avg_factor;
+    const double ct = P.cell_size > 0.0 ? P.cell_size : 2.0 * hmin * (1.0 + P.cell_slack);
+    const double block = ct * (double)T.f;
+    T.cmin = 1e300;
+    for (int a = 0; a < 3; ++a) {
+        const double L = up[a] - lo[a];
+        int n;
+        double c;
+        if (T.per[a]) {
+            n = (int)std::floor(L / block) * T.f;
+            if (n < T.f) n = T.f;
+            c = L / (double)n;
+        } else {
+            n = (int)std::ceil(L / block) * T.f;
+            if (n < T.f) n = T.f;
+            c = ct;
+        }
+        T.n[a] = n;
+        T.c[a] = c;
+        T.o[a] = lo[a];
+        T.ic[a] = 1.0 / c;
+        T.L[a] = L;
+        T.hi[a] = lo[a] + L;
+        T.cmin = std::min(T.cmin, c);
+        T.na[a] = n / T.f;
+    }
+    T.ncell = T.n[0] * T.n[1] * T.n[2];
+    T.navg = T.na[0] * T.na[1] * T.na[2];
+}
+
+Engine::Engine(const std::vector<dsh_params>& params)
+    : params_(params), staged_(params.size() * kMaxSpecies) {}
+
+Engine::~Engine() {
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    for (void* p : allocs_) (void)hipFree(p);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+int Engine::fail(int code, const std::string& msg) {
+    err_ = msg;
+    return code;
+}
+
+int Engine::hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return DSH_OK;
+    return fail(e == hipErrorOutOfMemory ? DSH_ERR_NOMEM : DSH_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T> int Engine::alloc(T** p, size_t count) {
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return hip_check(e, "hipMalloc");
+    allocs_.push_back(q);
+    dev_bytes_ += bytes;
+    *p = static_cast<T*>(q);
+    return DSH_OK;
+}
+
+long long Engine::particles(int tube) const {
+    long long n = 0;
+    for (int s = 0; s < kMaxSpecies; ++s) n += staged_[(size_t)tube * kMaxSpecies + s].n;
+    return n;
+}
+
+int Engine::upload(int tube, int species, const dsh_soa_view* v, int pm) {
+    if (prepared_) return fail(DSH_ERR_STATE, "upload after the first step");
+    if (tube < 0 || tube >= ntubes()) return fail(DSH_ERR_ARG, "tube index out of range");
+    const dsh_params& P = params_[tube];
+    if (species < 0 || species > P.n_dust) return fail(DSH_ERR_ARG, "species index out of range");
+    if (!v || pm < -1) return fail(DSH_ERR_ARG, "null view or negative count");
+    if (!v->x || !v->y || !v->z || !v->m || !v->ind) return fail(DSH_ERR_ARG, "view needs x, y, z, m and ind");
+    if (P.h_mode == 1 && !v->h && !v->rho) return fail(DSH_ERR_ARG, "h_mode 1 needs h or rho");
+    const int n = pm + 1;
+    HostSpecies& q = staged_[(size_t)tube * kMaxSpecies + species];
+    q.n = n;
+    q.set = true;
+    auto take = [n](std::vector<double>& dst, const double* src, double dflt) {
+        dst.resize(n);
+        for (int i = 0; i < n; ++i) dst[i] = src ? src[i] : dflt;
+    };
+    take(q.x, v->x, 0.0);
+    take(q.y, v->y, 0.0);
+    take(q.z, v->z, 0.0);
+    take(q.vx, v->vx, 0.0);
+    take(q.vy, v->vy, 0.0);
+    take(q.vz, v->vz, 0.0);
+    take(q.m, v->m, 0.0);
+    take(q.rho, v->rho, 0.0);
+    take(q.p, v->p, 0.0);
+    take(q.e, v->e, 0.0);
+    take(q.ts, v->t_stop, 0.0);
+    q.ind.assign(v->ind, v->ind + n);
+    q.h.resize(n);
+    for (int i = 0; i < n; ++i) {
+        if (v->h)
+            q.h[i] = v->h[i];
+        else if (P.h_mode == 1)
+            q.h[i] = P.eta * std::cbrt(q.m[i] / q.rho[i]);
+        else
+            q.h[i] = P.h;
+    }
+    return DSH_OK;
+}
+
+int Engine::prepare() {
+    if (prepared_) return DSH_OK;
+    const int T = ntubes();
+    if (T == 0) return fail(DSH_ERR_ARG, "no tubes");
+    precision_ = params_[0].precision;
+    for (int t = 0; t < T; ++t) {
+        const dsh_params& P = params_[t];
+        if (P.precision != precision_) return fail(DSH_ERR_ARG, "tubes of one batch must share the precision");
+        if (P.n_dust < 1 || P.n_dust > kMaxDust) return fail(DSH_ERR_ARG, "n_dust must be 1 or 2");
+        if (P.drag_mode != 1) return fail(DSH_ERR_UNSUPPORTED, "only drag_mode 1 (t_stop = rho_d/K) is built");
+        for (int s = 0; s <= P.n_dust; ++s)
+            if (!staged_[(size_t)t * kMaxSpecies + s].set) return fail(DSH_ERR_STATE, "species not uploaded");
+    }
+    tc_h_.assign(T, TubeConst{});
+    for (int t = 0; t < T; ++t) {
+        const dsh_params& P = params_[t];
+        TubeConst& C = tc_h_[t];
+        double hmin = 1e300;
+        for (int s = 0; s <= P.n_dust; ++s) {
+            const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
+            for (int i = 0; i < q.n; ++i) hmin = std::min(hmin, q.h[i]);
+        }
+        build_grid(P, hmin, C);
+        C.nsp = 1 + P.n_dust;
+        C.gamma = P.gamma;
+        C.cs = P.cs;
+        C.alpha = P.alpha;
+        C.beta = P.beta;
+        C.eps = P.eps;
+        C.eta = P.eta;
+        for (int k = 0; k < kMaxDust; ++k) {
+            C.K[k] = P.K[k];
+            C.mu[k] = P.mass_ratio[k];
+            C.eps_fb[k] = P.eps_fallback[k];
+        }
+        C.t_fb = P.tstop_fallback;
+        C.tau = P.tau;
+        C.cfl = P.cfl;
+        C.t_end = P.t_end;
+        C.eos = P.eos_type;
+        C.h_mode = P.h_mode;
+        C.eps2_quirk = P.eps2_uses_count1;
+        C.dt_mode = P.dt_mode;
+        C.max_steps = P.max_steps;
+        C.precision = P.precision;
+        if ((long long)C.ncell * C.nsp > 2000000000ll) return fail(DSH_ERR_ARG, "grid too large");
+    }
+    // segments: species-major, then tube
+    pseg_.clear();
+    gseg_.clear();
+    cseg_.clear();
+    aseg_.clear();
+    long long poff = 0, coff = 0, aoff = 0;
+    for (int s = 0; s < kMaxSpecies; ++s)
+        for (int t = 0; t < T; ++t) {
+            TubeConst& C = tc_h_[t];
+            if (s >= C.nsp) {
+                C.p_off[s] = 0;
+                C.np[s] = 0;
+                C.cell_off[s] = 0;
+                continue;
+            }
+            const int n = staged_[(size_t)t * kMaxSpecies + s].n;
+            C.p_off[s] = (int)poff;
+            C.np[s] = n;
+            C.cell_off[s] = (int)coff;
+            pseg_.push_back(Seg{(int)poff, n, t, s});
+            if (s == 0) gseg_.push_back(Seg{(int)poff, n, t, s});
+            cseg_.push_back(Seg{(int)coff, C.ncell, t, s});
+            poff += n;
+            coff += C.ncell;
+            if (poff > 2000000000ll || coff > 2000000000ll) return fail(DSH_ERR_ARG, "batch too large");
+        }
+    for (int t = 0; t < T; ++t) {
+        tc_h_[t].avg_off = (int)aoff;
+        aseg_.push_back(Seg{(int)aoff, tc_h_[t].navg, t, 0});
+        aoff += tc_h_[t].navg;
+    }
+    ntot_ = poff;
+    int rc;
+    if ((rc = hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
+
+    // block maps
+    auto make_map = [&](const std::vector<Seg>& segs, int per_block, Map& m) -> int {
+        std::vector<int> blk(segs.size() + 1, 0);
+        for (size_t i = 0; i < segs.size(); ++i) blk[i + 1] = blk[i] + (segs[i].n + per_block - 1) / per_block;
+        Seg* dseg;
+        int* dblk;
+        int r;
+        if ((r = alloc(&dseg, segs.size()))) return r;
+        if ((r = alloc(&dblk, blk.size()))) return r;
+        if (!segs.empty() &&
+            (r = hip_check(hipMemcpy(dseg, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice),
+                           "hipMemcpy seg")))
+            return r;
+        if ((r = hip_check(hipMemcpy(dblk, blk.data(), blk.size() * sizeof(int), hipMemcpyHostToDevice),
+                           "hipMemcpy blk")))
+            return r;
+        m.seg = dseg;
+        m.blk = dblk;
+        m.nseg = (int)segs.size();
+        m.nblocks = blk.back();
+        return DSH_OK;
+    };
+    if ((rc = make_map(pseg_, 256, a_.pmap))) return rc;
+    if ((rc = make_map(gseg_, 256, a_.gmap))) return rc;
+    if ((rc = make_map(cseg_, 256, a_.cmap))) return rc;
+    if ((rc = make_map(aseg_, 4, a_.amap))) return rc;
+
+    // particle buffers (two copies of each persistent field)
+    PBuf& B = a_.B;
+    B.ntot = ntot_;
+    const size_t n2 = 2 * (size_t)ntot_;
+    double** dfields[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.m, &B.h, &B.e, &B.rho,
+                          &B.p, &B.ts, &B.ax, &B.ay, &B.az, &B.ed};
+    for (double** f : dfields)
+        if ((rc = alloc(f, n2))) return rc;
+    if ((rc = alloc(&B.ind, n2))) return rc;
+    if ((rc = alloc(&B.id, n2))) return rc;
+    int** ifields[] = {&a_.key, &a_.slot, &a_.tsrc, &a_.tid, &a_.tkey, &a_.wkey, &a_.ncount};
+    for (int** f : ifields)
+        if ((rc = alloc(f, (size_t)ntot_))) return rc;
+    if ((rc = alloc(&a_.pr2, (size_t)ntot_))) return rc;
+    if ((rc = alloc(&a_.csav, (size_t)ntot_))) return rc;
+    if ((rc = alloc(&a_.nhash, (size_t)ntot_))) return rc;
+    a_.ncells = (int)coff;
+    if ((rc = alloc(&a_.count, (size_t)coff))) return rc;
+    if ((rc = alloc(&a_.start, (size_t)coff + 1))) return rc;
+    if ((rc = alloc(&a_.cellh, (size_t)coff))) return rc;
+    if ((rc = alloc(&a_.reach, (size_t)coff))) return rc;
+    a_.scan_nb = scan_blocks((int)coff);
+    if ((rc = alloc(&a_.scan_bsum, (size_t)a_.scan_nb))) return rc;
+    if ((rc = alloc(&a_.part, 5 * (size_t)aoff))) return rc;
+    TubeConst* dtc;
+    TubeDyn* dtd;
+    if ((rc = alloc(&dtc, (size_t)T))) return rc;
+    if ((rc = alloc(&dtd, (size_t)T))) return rc;
+    if ((rc = alloc(&a_.cur, 1))) return rc;
+    a_.tc = dtc;
+    a_.td = dtd;
+    a_.ntubes = T;
+    if ((rc = hip_check(hipMemcpy(dtc, tc_h_.data(), sizeof(TubeConst) * T, hipMemcpyHostToDevice), "H2D tubes")))
+        return rc;
+    const int zero = 0;
+    if ((rc = hip_check(hipMemcpy(a_.cur, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D cur"))) return rc;
+
+    // upload into buffer 0, original order, id = index
+    for (int t = 0; t < T; ++t)
+        for (int s = 0; s < tc_h_[t].nsp; ++s) {
+            const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
+            const size_t off = (size_t)tc_h_[t].p_off[s];
+            const size_t n = (size_t)q.n;
+            if (n == 0) continue;
+            struct {
+                double* dst;
+                const std::vector<double>& src;
+            } cp[] = {{B.x, q.x},   {B.y, q.y},   {B.z, q.z}, {B.vx, q.vx}, {B.vy, q.vy}, {B.vz, q.vz},
+                      {B.m, q.m},   {B.h, q.h},   {B.e, q.e}, {B.rho, q.rho}, {B.p, q.p}, {B.ts, q.ts}};
+            for (auto& c : cp)
+                if ((rc = hip_check(hipMemcpy(c.dst + off, c.src.data(), n * sizeof(double), hipMemcpyHostToDevice),
+                                    "H2D field")))
+                    return rc;
+            if ((rc = hip_check(hipMemcpy(B.ind + off, q.ind.data(), n * sizeof(int), hipMemcpyHostToDevice),
+                                "H2D ind")))
+                return rc;
+            std::vector<int> ids(n);
+            std::iota(ids.begin(), ids.end(), 0);
+            if ((rc = hip_check(hipMemcpy(B.id + off, ids.data(), n * sizeof(int), hipMemcpyHostToDevice),
+                                "H2D id")))
+                return rc;
+            std::vector<double> zeros(n, 0.0);
+            for (double* d : {B.ax, B.ay, B.az, B.ed})
+                for (int b = 0; b < 2; ++b)
+                    if ((rc = hip_check(hipMemcpy(d + b * ntot_ + off, zeros.data(), n * sizeof(double),
+                                                  hipMemcpyHostToDevice),
+                                        "H2D zero")))
+                        return rc;
+        }
+    if ((rc = hip_check(launch_prepare(a_, stream_), "k_prepare"))) return rc;
+    if ((rc = hip_check(hipStreamSynchronize(stream_), "prepare sync"))) return rc;
+    prepared_ = true;
+    return refresh_dyn();
+}
+
+namespace {
+const char* const kStageNames[] = {"begin",   "key",     "scan",    "scatter", "rank_permute", "cellmax",
+                                   "radius",  "density", "force",   "idic",    "finalize",     "flip"};
+constexpr int kStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
+}  // namespace
+
+// Enqueue one step.  With timed, an event is recorded before the first and after every
+// stage on the step stream; the elapsed times are collected at the next sync().
+int Engine::enqueue_step(bool timed) {
+    hipEvent_t* ev = nullptr;
+    if (timed) {
+        const size_t base = events_.size();
+        events_.resize(base + kStages + 1);
+        for (size_t i = base; i < events_.size(); ++i)
+            if (hipEventCreate(&events_[i]) != hipSuccess) return fail(DSH_ERR_HIP, "hipEventCreate");
+        ev = events_.data() + base;
+        (void)hipEventRecord(ev[0], stream_);
+        ++pending_timed_;
+    }
+    for (int i = 0; i < kStages; ++i) {
+        hipError_t e = hipSuccess;
+        switch (i) {
+            case 0: e = launch_begin(a_, stream_); break;
+            case 1: e = launch_key(a_, stream_); break;
+            case 2: e = launch_scan(a_, stream_); break;
+            case 3: e = launch_scatter(a_, stream_); break;
+            case 4: e = launch_rank_permute(a_, stream_); break;
+            case 5: e = launch_cellmax(a_, stream_); break;
+            case 6: e = launch_radius(a_, stream_); break;
+            case 7: e = launch_density(a_, precision_, stats_, stream_); break;
+            case 8: e = launch_force(a_, precision_, stream_); break;
+            case 9: e = launch_idic(a_, stream_); break;
+            case 10: e = launch_finalize(a_, stream_); break;
+            case 11: e = launch_flip(a_, stream_); break;
+        }
+        if (e != hipSuccess) return hip_check(e, kStageNames[i]);
+        if (timed) (void)hipEventRecord(ev[i + 1], stream_);
+    }
+    return DSH_OK;
+}
+
+int Engine::collect_times() {
+    if (pending_timed_ == 0) return DSH_OK;
+    std::vector<double> tot(kStages, 0.0);
+    for (int k = 0; k < pending_timed_; ++k) {
+        hipEvent_t* ev = events_.data() + (size_t)k * (kStages + 1);
+        for (int i = 0; i < kStages; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) return fail(DSH_ERR_HIP, "event time");
+            tot[i] += ms;
+        }
+    }
+    ktimes_.clear();
+    for (int i = 0; i < kStages; ++i) ktimes_.emplace_back(kStageNames[i], tot[i] / pending_timed_);
+    timed_steps_ = pending_timed_;
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
+    events_.clear();
+    pending_timed_ = 0;
+    return DSH_OK;
+}
+
+int Engine::build_graph() {
+    if (graph_exec_) {
+        (void)hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    if (graph_) {
+        (void)hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+    int rc;
+    if ((rc = hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")))
+        return rc;
+    const int r = enqueue_step(false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream_, &g);
+    if (r) return r;
+    if ((rc = hip_check(e, "end capture"))) return rc;
+    graph_ = g;
+    if ((rc = hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate")))
+        return rc;
+    graph_dirty_ = false;
+    return DSH_OK;
+}
+
+int Engine::launch_steps(int nsteps) {
+    int rc;
+    if (!prepared_ && (rc = prepare())) return rc;
+    for (int k = 0; k < nsteps; ++k) {
+        if (timing_) {
+            if ((rc = enqueue_step(true))) return rc;
+        } else if (use_graph_) {
+            if (graph_dirty_ || !graph_exec_)
+                if ((rc = build_graph())) return rc;
+            if ((rc = hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch"))) return rc;
+        } else {
+            if ((rc = enqueue_step(false))) return rc;
+        }
+    }
+    return DSH_OK;
+}
+
+int Engine::sync() {
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize(stream_), "step sync"))) return rc;
+    if ((rc = collect_times())) return rc;
+    return refresh_dyn();
+}
+
+int Engine::run(int nsteps) {
+    int rc;
+    if ((rc = launch_steps(nsteps))) return rc;
+    if ((rc = sync())) return rc;
+    return error_status();
+}
+
+int Engine::refresh_dyn() {
+    dyn_h_.resize(ntubes());
+    return hip_check(hipMemcpy(dyn_h_.data(), a_.td, sizeof(TubeDyn) * ntubes(), hipMemcpyDeviceToHost), "D2H dyn");
+}
+
+int Engine::error_status() {
+    for (int t = 0; t < ntubes(); ++t) {
+        const int e = dyn_h_[t].err;
+        if (e & kErrGrid)
+            return fail(DSH_ERR_GRID, "tube " + std::to_string(t) +
+                                          ": smoothing length grew beyond the neighbour grid's search radius");
+        if (e & kErrNonFinite)
+            return fail(DSH_ERR_NONFINITE, "tube " + std::to_string(t) + ": non-finite state");
+    }
+    return DSH_OK;
+}
+
+int Engine::download(int tube, int species, const dsh_soa_view* v) {
+    if (tube < 0 || tube >= ntubes()) return fail(DSH_ERR_ARG, "tube index out of range");
+    if (!v) return fail(DSH_ERR_ARG, "null view");
+    const dsh_params& P = params_[tube];
+    if (species < 0 || species > P.n_dust) return fail(DSH_ERR_ARG, "species index out of range");
+    int rc;
+    if (!prepared_ && (rc = prepare())) return rc;
+    if ((rc = hip_check(hipStreamSynchronize(stream_), "download sync"))) return rc;
+    if ((rc = refresh_dyn())) return rc;
+    int cur = 0;
+    if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
+    const TubeConst& C = tc_h_[tube];
+    const TubeDyn& D = dyn_h_[tube];
+    const int derived = D.done ? D.done_cur : cur;
+    const size_t n = (size_t)C.np[species];
+    const size_t off = (size_t)C.p_off[species];
+    std::vector<int> ids(n), idd(n);
+    if ((rc = hip_check(hipMemcpy(ids.data(), a_.B.id + (size_t)cur * ntot_ + off, n * sizeof(int),
+                                  hipMemcpyDeviceToHost),
+                        "D2H id")))
+        return rc;
+    if ((rc = hip_check(hipMemcpy(idd.data(), a_.B.id + (size_t)derived * ntot_ + off, n * sizeof(int),
+                                  hipMemcpyDeviceToHost),
+                        "D2H id")))
+        return rc;
+    std::vector<double> tmp(n);
+    auto get = [&](double* out, const double* base, int buf, const std::vector<int>& order) -> int {
+        if (!out) return DSH_OK;
+        int r = hip_check(hipMemcpy(tmp.data(), base + (size_t)buf * ntot_ + off, n * sizeof(double),
+                                    hipMemcpyDeviceToHost),
+                          "D2H field");
+        if (r) return r;
+        for (size_t q = 0; q < n; ++q) out[order[q]] = tmp[q];
+        return DSH_OK;
+    };
+    const PBuf& B = a_.B;
+    struct {
+        double* out;
+        const double* base;
+        bool deriv;
+    } fs[] = {{v->x, B.x, false},   {v->y, B.y, false},   {v->z, B.z, false},    {v->vx, B.vx, false},
+              {v->vy, B.vy, false}, {v->vz, B.vz, false}, {v->m, B.m, false},    {v->rho, B.rho, false},
+              {v->p, B.p, false},   {v->e, B.e, false},   {v->h, B.h, false},    {v->t_stop, B.ts, false},
+              {v->ax, B.ax, true},  {v->ay, B.ay, true},  {v->az, B.az, true},   {v->edot, B.ed, true}};
+    for (auto& f : fs)
+        if ((rc = get(f.out, f.base, f.deriv ? derived : cur, f.deriv ? idd : ids))) return rc;
+    if (v->ind) {
+        std::vector<int> ti(n);
+        if ((rc = hip_check(hipMemcpy(ti.data(), B.ind + (size_t)cur * ntot_ + off, n * sizeof(int),
+                                      hipMemcpyDeviceToHost),
+                            "D2H ind")))
+            return rc;
+        for (size_t q = 0; q < n; ++q) v->ind[ids[q]] = ti[q];
+    }
+    if (v->cell || v->nn) {
+        std::vector<int> wk(n);
+        const bool stepped = D.steps > 0;
+        if (stepped &&
+            (rc = hip_check(hipMemcpy(wk.data(), a_.wkey + off, n * sizeof(int), hipMemcpyDeviceToHost), "D2H key")))
+            return rc;
+        for (size_t q = 0; q < n; ++q) {
+            const int id = idd[q];
+            if (v->cell) v->cell[id] = stepped ? wk[q] - C.cell_off[species] : -1;
+            if (v->nn) v->nn[id] = (stepped && q + 1 < n && wk[q + 1] == wk[q]) ? idd[q + 1] : -1;
+        }
+    }
+    return DSH_OK;
+}
+
+int Engine::neighbour_stats(int tube, int species, int* count, uint64_t* hash) {
+    if (tube < 0 || tube >= ntubes() || species < 0 || species >= tc_h_[tube].nsp)
+        return fail(DSH_ERR_ARG, "bad tube/species");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize(stream_), "stats sync"))) return rc;
+    if ((rc = refresh_dyn())) return rc;
+    int cur = 0;
+    if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
+    const TubeDyn& D = dyn_h_[tube];
+    const int derived = D.done ? D.done_cur : cur;
+    const size_t n = (size_t)tc_h_[tube].np[species];
+    const size_t off = (size_t)tc_h_[tube].p_off[species];
+    std::vector<int> ids(n), c(n);
+    std::vector<unsigned long long> h(n);
+    if ((rc = hip_check(hipMemcpy(ids.data(), a_.B.id + (size_t)derived * ntot_ + off, n * sizeof(int),
+                                  hipMemcpyDeviceToHost),
+                        "D2H id")))
+        return rc;
+    if ((rc = hip_check(hipMemcpy(c.data(), a_.ncount + off, n * sizeof(int), hipMemcpyDeviceToHost), "D2H cnt")))
+        return rc;
+    if ((rc = hip_check(hipMemcpy(h.data(), a_.nhash + off, n * sizeof(unsigned long long), hipMemcpyDeviceToHost),
+                        "D2H hash")))
+        return rc;
+    for (size_t q = 0; q < n; ++q) {
+        if (count) count[ids[q]] = c[q];
+        if (hash) hash[ids[q]] = (uint64_t)h[q];
+    }
+    return DSH_OK;
+}
+
+}  // namespace dsh

@@ -1,0 +1,87 @@
+// dsh_engine.h - host engine: a batch of independent tubes stepped on one GPU stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dsh.h"
+#include "dsh_kernels.h"
+#include "dsh_types.h"
+
+namespace dsh {
+
+struct HostSpecies {
+    int n = 0;
+    bool set = false;
+    std::vector<double> x, y, z, vx, vy, vz, m, h, e, rho, p, ts;
+    std::vector<int> ind;
+};
+
+// Grid rule shared by every entry point (the oracle restates it independently).
+void build_grid(const dsh_params& P, double hmin, TubeConst& T);
+
+class Engine {
+public:
+    explicit Engine(const std::vector<dsh_params>& params);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int upload(int tube, int species, const dsh_soa_view* v, int pm);
+    int prepare();
+    // Launch nsteps steps (no host synchronisation in between) and wait for them.
+    int run(int nsteps);
+    // Launch nsteps steps without waiting (serving loop); sync() waits.
+    int launch_steps(int nsteps);
+    int sync();
+    int download(int tube, int species, const dsh_soa_view* v);
+    int neighbour_stats(int tube, int species, int* count, uint64_t* hash);
+    int refresh_dyn();   // copy per-tube dynamic state to the host
+    const TubeDyn& dyn(int tube) const { return dyn_h_[tube]; }
+    const TubeConst& tube_const(int tube) const { return tc_h_[tube]; }
+    int ntubes() const { return (int)params_.size(); }
+    bool prepared() const { return prepared_; }
+    long long particles(int tube) const;
+    size_t device_bytes() const { return dev_bytes_; }
+
+    void set_stats(bool on) { stats_ = on; graph_dirty_ = true; }
+    void set_timing(bool on) { timing_ = on; }
+    void set_graph(bool on) { use_graph_ = on; }
+    // per-stage GPU time (ms) averaged over the steps launched with timing on
+    const std::vector<std::pair<std::string, double>>& kernel_times() const { return ktimes_; }
+    int timed_steps() const { return timed_steps_; }
+    const std::string& error() const { return err_; }
+    hipStream_t stream() const { return stream_; }
+    int error_status();   // DSH_OK or the first tube error as a status
+
+private:
+    int fail(int code, const std::string& msg);
+    int hip_check(hipError_t e, const char* what);
+    template <class T> int alloc(T** p, size_t count);
+    int enqueue_step(bool timed);
+    int build_graph();
+
+    std::vector<dsh_params> params_;
+    std::vector<HostSpecies> staged_;   // [tube * kMaxSpecies + species]
+    std::vector<TubeConst> tc_h_;
+    std::vector<TubeDyn> dyn_h_;
+    std::vector<Seg> pseg_, gseg_, cseg_, aseg_;
+    std::vector<void*> allocs_;
+    StepArgs a_{};
+    hipStream_t stream_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
+    int precision_ = 0;
+    long long ntot_ = 0;
+    size_t dev_bytes_ = 0;
+    std::vector<std::pair<std::string, double>> ktimes_;
+    std::vector<hipEvent_t> events_;   // (stages + 1) per timed step, pending until sync()
+    int pending_timed_ = 0, timed_steps_ = 0;
+    int collect_times();
+    std::string err_;
+};
+
+}  // namespace dsh

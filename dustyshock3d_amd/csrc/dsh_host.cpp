@@ -1,0 +1,253 @@
+// dsh_host.cpp - host-only parts of the C ABI: default parameters, the Init.txt reader
+// (kernel.cu:1705-1742) and the seeded shock-tube set-up used by the benchmark
+// configurations (the structure of init_Sod_X_Dust, kernel.cu:1391-1693).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/dsh.h"
+
+namespace {
+
+uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Counter-based stream keyed by (seed, tube, species, id, axis): the set-up of one
+// particle does not depend on how many others were generated before it.
+double jitter_unit(uint64_t seed, int tube, int species, int id, int axis) {
+    uint64_t z = mix64(seed ^ ((uint64_t)(uint32_t)tube * 0xD1B54A32D192ED03ull));
+    z = mix64(z ^ ((uint64_t)(uint32_t)(species * 4 + axis) * 0x8CB92BA72F3D8DD7ull));
+    z = mix64(z ^ ((uint64_t)(uint32_t)id * 0xA0761D6478BD642Full));
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    return 2.0 * u - 1.0;
+}
+
+double wrap_periodic(double y, double w) {
+    if (y < 0.0) y = y + w;
+    if (y >= w) y = y - w;
+    return y;
+}
+
+// Lattice of one tube: x <= 0 at spacing dx (rows anchored at x = 0 going -x, as
+// kernel.cu:1421), x > 0 at 2 dx starting at +2dx (kernel.cu:1461); the cross-section
+// holds 2 n and n points per axis, so both halves tile the periodic box exactly and
+// equal-mass particles give a density ratio of exactly 8.
+struct Lattice {
+    int nx[2], ny[2], nz[2];
+    double dx[2], dy[2], dz[2];
+    double m, rho[2];
+};
+
+Lattice lattice(const dsh_tube_spec& t) {
+    Lattice L{};
+    L.dx[0] = t.dx_left;
+    L.dx[1] = 2.0 * t.dx_left;
+    L.nx[0] = (int)std::floor((0.0 - t.x_min) / L.dx[0] + 0.5);
+    L.nx[1] = (int)std::floor((t.x_max - 0.0) / L.dx[1] + 0.5);
+    L.ny[1] = std::max(1, (int)std::floor(t.y_width / L.dx[1] + 0.5));
+    L.nz[1] = std::max(1, (int)std::floor(t.z_width / L.dx[1] + 0.5));
+    L.ny[0] = 2 * L.ny[1];
+    L.nz[0] = 2 * L.nz[1];
+    for (int s = 0; s < 2; ++s) {
+        L.dy[s] = t.y_width / L.ny[s];
+        L.dz[s] = t.z_width / L.nz[s];
+    }
+    L.m = t.rho_left * (L.dx[0] * L.dy[0] * L.dz[0]);
+    L.rho[0] = t.rho_left;
+    L.rho[1] = L.m / (L.dx[1] * L.dy[1] * L.dz[1]);
+    return L;
+}
+
+bool spec_ok(const dsh_tube_spec* t) {
+    return t && t->dx_left > 0 && t->x_max > 0 && t->x_min < 0 && t->y_width > 0 && t->z_width > 0 &&
+           t->rho_left > 0 && t->dust_to_gas > 0 && t->K > 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsh_default_params(dsh_params* p) {
+    if (!p) return DSH_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    // the reference Init.txt (Init.txt:2-18) and constants (data.h:6-7, kernel.cu:1498-1502,1744)
+    p->x_min = -2.0;
+    p->x_max = 2.0;
+    p->y_min = -2.0;
+    p->y_max = 2.0;
+    p->z_min = -2.0;
+    p->z_max = 2.0;
+    p->eos_type = 0;
+    p->max_particles = 1000000;
+    p->particles_per_length = 40;
+    p->h = 0.1;
+    p->avg_cell_coeff = 0.5;
+    p->tau = 5.0e-3;
+    p->t_end = 0.2;
+    p->t_out = 0.1;
+    p->alpha = 1.0;
+    p->beta = 2.0;
+    p->eps = 0.01;
+    p->gamma = 1.4;
+    p->cs = 1.0;
+    p->h_mode = 0;
+    p->eta = 1.2;
+    p->n_dust = 2;
+    p->drag_mode = 1;
+    p->K[0] = p->K[1] = 1.0;
+    p->R_dust[0] = 0.01;
+    p->R_dust[1] = 1.0;
+    p->mass_ratio[0] = 0.75;
+    p->mass_ratio[1] = 0.5;
+    p->eps_fallback[0] = 0.5;
+    p->eps_fallback[1] = 0.33;
+    p->tstop_fallback = 1.0;
+    p->eps2_uses_count1 = 1;
+    p->dt_mode = DSH_DT_FIXED;
+    p->cfl = 0.3;
+    p->max_steps = 0;
+    p->cell_size = 0.0;
+    p->cell_slack = 0.05;
+    p->avg_factor = 2;
+    p->precision = DSH_FP64;
+    return DSH_OK;
+}
+
+int dsh_tube_params(const dsh_tube_spec* t, dsh_params* p) {
+    if (!spec_ok(t) || !p) return DSH_ERR_ARG;
+    dsh_default_params(p);
+    p->x_min = t->x_min;
+    p->x_max = t->x_max;
+    p->y_min = 0.0;
+    p->y_max = t->y_width;
+    p->z_min = 0.0;
+    p->z_max = t->z_width;
+    p->eos_type = 1;
+    p->max_particles = 0;
+    p->particles_per_length = 1.0 / t->dx_left;
+    p->h = 1.2 * t->dx_left;
+    p->avg_cell_coeff = 0.0;
+    p->tau = 0.0;
+    p->t_end = t->t_end;
+    p->t_out = t->t_end;
+    p->periodic_y = 1;
+    p->periodic_z = 1;
+    p->h_mode = 1;
+    p->n_dust = 1;
+    p->K[0] = p->K[1] = t->K;
+    p->mass_ratio[0] = p->mass_ratio[1] = t->dust_to_gas;
+    p->eps2_uses_count1 = 0;
+    p->dt_mode = t->dt_mode;
+    p->cfl = t->cfl;
+    p->max_steps = t->max_steps;
+    p->avg_factor = 2;
+    p->precision = t->precision;
+    return DSH_OK;
+}
+
+int dsh_tube_count(const dsh_tube_spec* t, int* n) {
+    if (!spec_ok(t) || !n) return DSH_ERR_ARG;
+    const Lattice L = lattice(*t);
+    const long long c = (long long)L.nx[0] * L.ny[0] * L.nz[0] + (long long)L.nx[1] * L.ny[1] * L.nz[1];
+    if (c > 2000000000ll) return DSH_ERR_ARG;
+    n[0] = (int)c;
+    n[1] = (int)c;
+    n[2] = 0;
+    return DSH_OK;
+}
+
+int dsh_tube_ic(const dsh_tube_spec* t, int species, const dsh_soa_view* v) {
+    if (!spec_ok(t) || !v || species < 0 || species > 1) return DSH_ERR_ARG;
+    const Lattice L = lattice(*t);
+    const double ratio = species == 0 ? 1.0 : t->dust_to_gas;
+    const double cs = 1.0, gam = 1.4;
+    int id = 0;
+    for (int side = 0; side < 2; ++side) {
+        const double dx = L.dx[side], dy = L.dy[side], dz = L.dz[side];
+        const double hloc = 1.2 * std::cbrt(dx * dy * dz);
+        const double rho = L.rho[side] * ratio;
+        for (int i = 0; i < L.nx[side]; ++i)
+            for (int j = 0; j < L.ny[side]; ++j)
+                for (int k = 0; k < L.nz[side]; ++k, ++id) {
+                    double x = side == 0 ? 0.0 - (double)i * dx : dx + (double)i * dx;
+                    double y = 0.0 + (double)j * dy;
+                    double z = 0.0 + (double)k * dz;
+                    x = x + t->jitter * dx * jitter_unit(t->seed, t->tube_index, species, id, 0);
+                    y = y + t->jitter * dy * jitter_unit(t->seed, t->tube_index, species, id, 1);
+                    z = z + t->jitter * dz * jitter_unit(t->seed, t->tube_index, species, id, 2);
+                    y = wrap_periodic(y, t->y_width);
+                    z = wrap_periodic(z, t->z_width);
+                    if (v->x) v->x[id] = x;
+                    if (v->y) v->y[id] = y;
+                    if (v->z) v->z[id] = z;
+                    if (v->vx) v->vx[id] = 0.0;
+                    if (v->vy) v->vy[id] = 0.0;
+                    if (v->vz) v->vz[id] = 0.0;
+                    if (v->ax) v->ax[id] = 0.0;
+                    if (v->ay) v->ay[id] = 0.0;
+                    if (v->az) v->az[id] = 0.0;
+                    if (v->m) v->m[id] = L.m * ratio;
+                    if (v->rho) v->rho[id] = rho;
+                    if (v->p) v->p[id] = species == 0 ? cs * cs * rho : 0.0;
+                    if (v->e) v->e[id] = species == 0 ? cs * cs / (gam - 1.0) : 0.0;
+                    if (v->h) v->h[id] = hloc;
+                    if (v->t_stop) v->t_stop[id] = species == 0 ? 0.0 : rho / t->K;
+                    if (v->edot) v->edot[id] = 0.0;
+                    if (v->ind)
+                        v->ind[id] = (x < t->x_min + t->frozen_h * hloc || x > t->x_max - t->frozen_h * hloc) ? 1 : 0;
+                    if (v->nn) v->nn[id] = -1;
+                    if (v->cell) v->cell[id] = -1;
+                }
+    }
+    return DSH_OK;
+}
+
+// Init.txt: a comment line, then one value per line (kernel.cu:1705-1722).  Lines after
+// the 18th, when present, are extension values in the order of dsh_params.
+int dsh_read_init(const char* path, dsh_params* p) {
+    if (!path || !p) return DSH_ERR_ARG;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return DSH_ERR_IO;
+    dsh_default_params(p);
+    char s[256];
+    int line = 0, ok = 1;
+    double* dbl1[] = {&p->x_min, &p->x_max, &p->y_min, &p->y_max, &p->z_min, &p->z_max};
+    if (!std::fgets(s, sizeof s, f)) ok = 0;
+    for (int i = 0; ok && i < 6; ++i) ok = std::fgets(s, sizeof s, f) && std::sscanf(s, "%lf", dbl1[i]) == 1;
+    if (ok) ok = std::fgets(s, sizeof s, f) && std::sscanf(s, "%d", &p->eos_type) == 1;
+    if (ok) ok = std::fgets(s, sizeof s, f) && std::sscanf(s, "%d", &p->max_particles) == 1;
+    double* dbl2[] = {&p->particles_per_length, &p->h, &p->avg_cell_coeff, &p->tau, &p->t_end,
+                      &p->t_out, &p->alpha, &p->beta, &p->eps};
+    for (int i = 0; ok && i < 9; ++i) ok = std::fgets(s, sizeof s, f) && std::sscanf(s, "%lf", dbl2[i]) == 1;
+    line = 18;
+    if (!ok) {
+        std::fclose(f);
+        return DSH_ERR_IO;
+    }
+    // optional extension lines
+    struct Ext {
+        char kind;
+        void* dst;
+    } ext[] = {{'i', &p->periodic_y},     {'i', &p->periodic_z},  {'d', &p->gamma},        {'d', &p->cs},
+               {'i', &p->h_mode},         {'d', &p->eta},         {'i', &p->n_dust},       {'i', &p->drag_mode},
+               {'d', &p->K[0]},           {'d', &p->K[1]},        {'d', &p->mass_ratio[0]}, {'d', &p->mass_ratio[1]},
+               {'d', &p->eps_fallback[0]}, {'d', &p->eps_fallback[1]}, {'d', &p->tstop_fallback},
+               {'i', &p->eps2_uses_count1}, {'i', &p->dt_mode},   {'d', &p->cfl},          {'i', &p->max_steps},
+               {'d', &p->cell_size},      {'d', &p->cell_slack},  {'i', &p->avg_factor},   {'i', &p->precision}};
+    for (auto& e : ext) {
+        if (!std::fgets(s, sizeof s, f)) break;
+        const int r = e.kind == 'i' ? std::sscanf(s, "%d", (int*)e.dst) : std::sscanf(s, "%lf", (double*)e.dst);
+        if (r != 1) break;
+        ++line;
+    }
+    std::fclose(f);
+    (void)line;
+    return DSH_OK;
+}
+
+}  // extern "C"

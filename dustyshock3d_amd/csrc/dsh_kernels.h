@@ -1,0 +1,58 @@
+// dsh_kernels.h - host-side launchers of the gfx950 kernels (dsh_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dsh_types.h"
+
+namespace dsh {
+
+// Block map of one launch: segment s owns blocks [blk[s], blk[s+1]).
+struct Map {
+    const Seg* seg;
+    const int* blk;
+    int nseg;
+    int nblocks;
+};
+
+struct StepArgs {
+    const TubeConst* tc;
+    TubeDyn* td;
+    int* cur;              // device: buffer index of the current state
+    PBuf B;
+    int ntubes;
+    // per particle (global sorted space)
+    int *key, *slot, *tsrc, *tid, *tkey, *wkey;
+    double *pr2, *csav;
+    int* ncount;
+    unsigned long long* nhash;
+    // per neighbour cell
+    int *count, *start;
+    double* cellh;
+    double* reach;
+    int ncells;
+    int* scan_bsum;
+    int scan_nb;
+    // per averaging cell
+    double* part;          // 5 per averaging cell
+    // launch maps
+    Map pmap, gmap, cmap, amap;
+};
+
+hipError_t launch_prepare(const StepArgs& a, hipStream_t s);
+hipError_t launch_begin(const StepArgs& a, hipStream_t s);
+hipError_t launch_key(const StepArgs& a, hipStream_t s);
+hipError_t launch_scan(const StepArgs& a, hipStream_t s);
+hipError_t launch_scatter(const StepArgs& a, hipStream_t s);
+hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s);
+hipError_t launch_cellmax(const StepArgs& a, hipStream_t s);
+hipError_t launch_radius(const StepArgs& a, hipStream_t s);
+hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s);
+hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s);
+hipError_t launch_idic(const StepArgs& a, hipStream_t s);
+hipError_t launch_finalize(const StepArgs& a, hipStream_t s);
+hipError_t launch_flip(const StepArgs& a, hipStream_t s);
+
+int scan_blocks(int n);
+
+}  // namespace dsh

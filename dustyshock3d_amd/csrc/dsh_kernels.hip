@@ -1,0 +1,1063 @@
+// dsh_kernels.hip - gfx950 kernels of one dusty-gas SPH step over a batch of tubes.
+//
+// Step (one stream, no host round trips):
+//   begin        zero cell counts and per-tube maxima
+//   key          neighbour-cell key per particle; wave-aggregated atomic slot in its cell
+//   scan         exclusive scan of cell counts -> cell starts (global over all segments)
+//   scatter      particle -> (cell start + slot)
+//   rank_permute rank inside the cell by particle id, gather the state into the work buffer
+//                (the sorted order is (cell key, id): the reference's linked-list order)
+//   cellmax      max h per cell, per (species, tube)
+//   radius       per-cell search radius in cells, from a window max of cell h
+//   density      same-species density sums; pressure, sound speed, stopping time
+//   force        gas pressure + artificial-viscosity acceleration and de/dt
+//   idic         per averaging cell: sums, implicit two-fluid drag solve (FP64), per-particle
+//                velocity update, drift, energy, smoothing length, conservation partials
+//   finalize     per tube: conservation sums, time, next CFL step, completion
+//   flip         swap state and work buffers
+#include "dsh_kernels.h"
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dsh {
+namespace {
+
+constexpr int kBS = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBS * kScanItems;
+constexpr double kInvPi = 1.0 / 3.14159265;   // kernel.cu:13-15
+
+__device__ __forceinline__ int find_seg(const Map& mp, int b) {
+    int lo = 0, hi = mp.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (mp.blk[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int floordiv(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
+
+__device__ __forceinline__ int axis_cell(double x, double o, double ic, int n) {
+    const double t = (x - o) * ic;
+    if (!(t >= 0.0)) return 0;
+    if (t >= (double)n) return n - 1;
+    return (int)t;
+}
+
+__device__ __forceinline__ double wrap(double y, double lo, double L, double hi) {
+    if (y < lo) y = y + L;
+    if (y >= hi) y = y - L;
+    return y;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ uint64_t pair_tag(int id, int sy, int sz) {
+    return mix64(((uint64_t)(uint32_t)id << 8) ^ ((uint64_t)(sy + 8) << 4) ^ (uint64_t)(sz + 8));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------- real-type helpers
+template <typename R> __device__ __forceinline__ R rcp(R x);
+template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename R> __device__ __forceinline__ R sqrt_r(R x);
+template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_sqrtf(x); }
+
+// Plain sum in FP64, Kahan-compensated sum in FP32.
+template <typename R> struct Acc;
+template <> struct Acc<double> {
+    double s = 0.0;
+    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ double get() const { return s; }
+};
+template <> struct Acc<float> {
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
+};
+
+// Cubic spline W(r,h) (kernel.cu:11-22) and dW/dr for r >= 0 (kernel.cu:24-38),
+// written with one reciprocal of h per pair.
+template <typename R> __device__ __forceinline__ R spline_w(R q) {
+    if (q < R(1)) return R(1) - R(1.5) * q * q + R(0.75) * q * q * q;
+    if (q <= R(2)) {
+        const R u = R(2) - q;
+        return R(0.25) * u * u * u;
+    }
+    return R(0);
+}
+template <typename R> __device__ __forceinline__ R spline_dw(R q) {
+    if (q <= R(1)) return R(-3) * q + R(2.25) * q * q;
+    if (q <= R(2)) {
+        const R u = R(2) - q;
+        return R(-0.75) * u * u;
+    }
+    return R(0);
+}
+
+// ---------------------------------------------------------------- neighbour traversal
+// Visits every candidate j within ceil(reach / c_a) cells of (kx,ky,kz) along each axis a,
+// periodic in y/z (each offset is a distinct (cell, image) pair, so thin tubes get every
+// image in reach), clamped in x, and calls f.pair for
+// d2 < (h_i + h_j)^2 with d2 = fma(dz,dz,fma(dy,dy,dx*dx)) in FP64.  Rows of cells along
+// x are contiguous in the sorted arrays, so each (dy,dz) offset is one index range.
+template <class F>
+__device__ __forceinline__ void for_pairs(const TubeConst& T, int cell_off, const int* __restrict__ start,
+                                          int kx, int ky, int kz, double reach, const double* __restrict__ X,
+                                          const double* __restrict__ Y, const double* __restrict__ Z,
+                                          const double* __restrict__ H, double xi, double yi, double zi,
+                                          double hi, F& f) {
+    const int nx = T.n[0], ny = T.n[1], nz = T.n[2];
+    const int Rx = (int)ceil(reach * T.ic[0]);
+    const int Ry = (int)ceil(reach * T.ic[1]);
+    const int Rz = (int)ceil(reach * T.ic[2]);
+    const int x0 = max(kx - Rx, 0), x1 = min(kx + Rx, nx - 1);
+    for (int oz = -Rz; oz <= Rz; ++oz) {
+        int cz = kz + oz, sz = 0;
+        if (T.per[2]) {
+            sz = floordiv(cz, nz);
+            cz -= sz * nz;
+        } else if (cz < 0 || cz >= nz) {
+            continue;
+        }
+        const double shz = (double)sz * T.L[2];
+        for (int oy = -Ry; oy <= Ry; ++oy) {
+            int cy = ky + oy, sy = 0;
+            if (T.per[1]) {
+                sy = floordiv(cy, ny);
+                cy -= sy * ny;
+            } else if (cy < 0 || cy >= ny) {
+                continue;
+            }
+            const double shy = (double)sy * T.L[1];
+            const int row = cell_off + nx * (cy + ny * cz);
+            const int jb = start[row + x0], je = start[row + x1 + 1];
+            for (int j = jb; j < je; ++j) {
+                const double dx = X[j] - xi;
+                const double dy = (Y[j] - yi) + shy;
+                const double dz = (Z[j] - zi) + shz;
+                const double d2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                const double hs = hi + H[j];
+                if (d2 < hs * hs) f.pair(j, dx, dy, dz, d2, sy, sz);
+            }
+        }
+    }
+}
+
+struct ItemPos {
+    Seg sg;
+    int li;
+};
+
+__device__ __forceinline__ ItemPos locate(const Map& mp, int per_block) {
+    const int s = find_seg(mp, blockIdx.x);
+    ItemPos ip;
+    ip.sg = mp.seg[s];
+    ip.li = (blockIdx.x - mp.blk[s]) * per_block;
+    return ip;
+}
+
+// ---------------------------------------------------------------- kernels
+
+// Initial time step and conservation sums from the uploaded state (buffer 0).
+__global__ __launch_bounds__(kBS) void k_prepare(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                 PBuf B) {
+    __shared__ double sh[5][kBS / 64];
+    const int t = blockIdx.x;
+    const TubeConst& T = tc[t];
+    double mn = 1e300, px = 0, py = 0, pz = 0, E = 0;
+    for (int s = 0; s < T.nsp; ++s) {
+        const int off = T.p_off[s], n = T.np[s];
+        for (int i = threadIdx.x; i < n; i += kBS) {
+            const int p = off + i;
+            const double vx = B.vx[p], vy = B.vy[p], vz = B.vz[p], m = B.m[p];
+            px += m * vx;
+            py += m * vy;
+            pz += m * vz;
+            double k = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+            if (s == 0) k += m * B.e[p];
+            E += k;
+            if (B.ind[p] == 0) {
+                const double v = sqrt(vx * vx + vy * vy + vz * vz);
+                const double r = B.h[p] / (T.cs + v);
+                mn = fmin(mn, r);
+            }
+        }
+    }
+    px = wave_sum(px);
+    py = wave_sum(py);
+    pz = wave_sum(pz);
+    E = wave_sum(E);
+    mn = wave_min(mn);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][w] = px;
+        sh[1][w] = py;
+        sh[2][w] = pz;
+        sh[3][w] = E;
+        sh[4][w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a[4] = {0, 0, 0, 0}, m2 = 1e300;
+        for (int k = 0; k < kBS / 64; ++k) {
+            for (int c = 0; c < 4; ++c) a[c] += sh[c][k];
+            m2 = fmin(m2, sh[4][k]);
+        }
+        TubeDyn& D = td[t];
+        D.t = 0.0;
+        D.dt = (T.dt_mode == 0) ? T.tau : T.cfl * m2;
+        D.steps = 0;
+        D.done = 0;
+        D.final_step = 0;
+        D.err = 0;
+        D.done_cur = 0;
+        for (int c = 0; c < 4; ++c) D.cons[c] = a[c];
+        for (int s = 0; s < kMaxSpecies; ++s) D.hmax_bits[s] = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(kBS) void k_begin(int* __restrict__ count, int ncells, TubeDyn* __restrict__ td,
+                                               int ntubes) {
+    const int i = blockIdx.x * kBS + threadIdx.x;
+    if (i < ncells) count[i] = 0;
+    if (i < ntubes) {
+        for (int s = 0; s < kMaxSpecies; ++s) td[i].hmax_bits[s] = 0ull;
+    }
+}
+
+// Cell key and slot.  Consecutive lanes hold consecutive particles of the previous
+// step's sorted order, so equal keys come in runs: one atomic per run, found with a
+// wave ballot of run heads.
+__global__ __launch_bounds__(kBS) void k_key(Map mp, const TubeConst* __restrict__ tc, const int* __restrict__ cur,
+                                             PBuf B, int* __restrict__ key, int* __restrict__ slot,
+                                             int* __restrict__ count) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    const bool valid = li < ip.sg.n;
+    const int p = ip.sg.off + li;
+    const TubeConst& T = tc[ip.sg.tube];
+    const long long so = (long long)(*cur) * B.ntot;
+    int k = -1;
+    if (valid) {
+        const int kx = axis_cell(B.x[so + p], T.o[0], T.ic[0], T.n[0]);
+        const int ky = axis_cell(B.y[so + p], T.o[1], T.ic[1], T.n[1]);
+        const int kz = axis_cell(B.z[so + p], T.o[2], T.ic[2], T.n[2]);
+        k = T.cell_off[ip.sg.species] + kx + T.n[0] * (ky + T.n[1] * kz);
+    }
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(k, 1, 64);
+    const bool head = valid && (lane == 0 || prev != k);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long vmask = __ballot(valid);
+    const int nvalid = __popcll(vmask);
+    const unsigned long long upto = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
+    const int h0 = upto ? (63 - __clzll(upto)) : 0;
+    const unsigned long long after = (lane == 63) ? 0ull : (heads & ~((2ull << lane) - 1ull));
+    int next = after ? (__ffsll((long long)after) - 1) : 64;
+    if (next > nvalid) next = nvalid;
+    int base = 0;
+    if (head) base = atomicAdd(&count[k], next - lane);
+    base = __shfl(base, h0, 64);
+    if (valid) {
+        key[p] = k;
+        slot[p] = base + (lane - h0);
+    }
+}
+
+// ---- exclusive scan of cell counts (three passes) ----
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+    __shared__ int ws[kBS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    int wo = 0, tot = 0;
+    for (int k = 0; k < kBS / 64; ++k) {
+        if (k < w) wo += ws[k];
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return wo + x - v;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ bsum) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < n) s += in[base + k];
+    int tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_top(int* __restrict__ bsum, int nb) {
+    int carry = 0;
+    for (int base = 0; base < nb; base += kBS) {
+        const int i = base + threadIdx.x;
+        const int v = (i < nb) ? bsum[i] : 0;
+        int tot;
+        const int ex = block_exclusive_scan(v, &tot);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(kBS) void k_scan_down(const int* __restrict__ in, int n, const int* __restrict__ bsum,
+                                                   int* __restrict__ out) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int tot;
+    int run = block_exclusive_scan(s, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (base + kScanItems >= n && base < n) out[n] = run;  // the thread holding the last item
+}
+
+__global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__ cur, PBuf B,
+                                                 const int* __restrict__ key, const int* __restrict__ slot,
+                                                 const int* __restrict__ start, int* __restrict__ tsrc,
+                                                 int* __restrict__ tid, int* __restrict__ tkey) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const int p = ip.sg.off + li;
+    const long long so = (long long)(*cur) * B.ntot;
+    const int k = key[p];
+    const int pos = start[k] + slot[p];
+    tsrc[pos] = p;
+    tid[pos] = B.id[so + p];
+    tkey[pos] = k;
+}
+
+// Final position = cell start + number of cell members with a smaller id.
+__global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const int* __restrict__ cur, PBuf B,
+                                                      const int* __restrict__ start, const int* __restrict__ tsrc,
+                                                      const int* __restrict__ tid, const int* __restrict__ tkey,
+                                                      int* __restrict__ wkey) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const int q = ip.sg.off + li;
+    const int c = tkey[q];
+    const int lo = start[c], hi = start[c + 1];
+    const int my = tid[q];
+    int r = 0;
+    for (int k = lo; k < hi; ++k) r += (tid[k] < my) ? 1 : 0;
+    const int dst = lo + r;
+    const int src = tsrc[q];
+    const long long so = (long long)(*cur) * B.ntot;
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    B.x[wo + dst] = B.x[so + src];
+    B.y[wo + dst] = B.y[so + src];
+    B.z[wo + dst] = B.z[so + src];
+    B.vx[wo + dst] = B.vx[so + src];
+    B.vy[wo + dst] = B.vy[so + src];
+    B.vz[wo + dst] = B.vz[so + src];
+    B.m[wo + dst] = B.m[so + src];
+    B.h[wo + dst] = B.h[so + src];
+    B.rho[wo + dst] = B.rho[so + src];
+    B.ind[wo + dst] = B.ind[so + src];
+    B.id[wo + dst] = my;
+    if (ip.sg.species == 0) {
+        B.e[wo + dst] = B.e[so + src];
+        B.p[wo + dst] = B.p[so + src];
+    } else {
+        B.ts[wo + dst] = B.ts[so + src];
+    }
+    wkey[dst] = c;
+}
+
+__global__ __launch_bounds__(kBS) void k_cellmax(Map mp, const int* __restrict__ cur, PBuf B,
+                                                 const int* __restrict__ start, double* __restrict__ cellh,
+                                                 TubeDyn* __restrict__ td) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    double hm = 0.0;
+    if (li < ip.sg.n) {
+        const int c = ip.sg.off + li;
+        const long long wo = (long long)(1 - *cur) * B.ntot;
+        const int lo = start[c], hi = start[c + 1];
+        for (int k = lo; k < hi; ++k) hm = fmax(hm, B.h[wo + k]);
+        cellh[c] = hm;
+    }
+    hm = wave_max(hm);
+    if ((threadIdx.x & 63) == 0 && hm > 0.0)
+        atomicMax(&td[ip.sg.tube].hmax_bits[ip.sg.species], (unsigned long long)__double_as_longlong(hm));
+}
+
+// Reach of a cell: every neighbour j of a particle i of the cell has |x_i - x_j| < h_i + h_j
+// <= h_cell + h_j.  Cells more than W_a = ceil(2 h_max (1+m) / c_a) cells away along an
+// axis are at least W_a c_a >= 2 h_max apart, so h_j is bounded by the max of cell h over
+// that window (a periodic axis shorter than the window is scanned once), and the pair
+// kernels search ceil(reach / c_a) cells along each axis.
+__global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                const double* __restrict__ cellh, double* __restrict__ reach) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const int c = ip.sg.off + li;
+    const TubeConst& T = tc[ip.sg.tube];
+    TubeDyn& D = td[ip.sg.tube];
+    const double hc = cellh[c];
+    if (hc <= 0.0) {
+        reach[c] = 0.0;
+        return;
+    }
+    const double hs = __longlong_as_double((long long)D.hmax_bits[ip.sg.species]);
+    const double span = 2.0 * hs * (1.0 + kRadiusMargin);
+    int w[3];
+    for (int a = 0; a < 3; ++a) {
+        w[a] = (int)ceil(span * T.ic[a]);
+        if (w[a] > kMaxCellReach) {
+            if (li == 0) atomicOr(&D.err, kErrGrid);
+            reach[c] = 0.0;
+            return;
+        }
+    }
+    if (T.h_mode == 0) {
+        reach[c] = span;
+        return;
+    }
+    const int kx = li % T.n[0];
+    const int ky = (li / T.n[0]) % T.n[1];
+    const int kz = li / (T.n[0] * T.n[1]);
+    const int base = ip.sg.off;
+    // a periodic axis whose window covers every cell is scanned once
+    const int lz = (T.per[2] && 2 * w[2] + 1 >= T.n[2]) ? 0 : -w[2];
+    const int uz = (T.per[2] && 2 * w[2] + 1 >= T.n[2]) ? T.n[2] - 1 : w[2];
+    const int ly = (T.per[1] && 2 * w[1] + 1 >= T.n[1]) ? 0 : -w[1];
+    const int uy = (T.per[1] && 2 * w[1] + 1 >= T.n[1]) ? T.n[1] - 1 : w[1];
+    const bool absz = T.per[2] && 2 * w[2] + 1 >= T.n[2];
+    const bool absy = T.per[1] && 2 * w[1] + 1 >= T.n[1];
+    double wm = 0.0;
+    for (int oz = lz; oz <= uz; ++oz) {
+        int cz = absz ? oz : kz + oz;
+        if (!absz) {
+            if (T.per[2]) cz -= floordiv(cz, T.n[2]) * T.n[2];
+            else if (cz < 0 || cz >= T.n[2]) continue;
+        }
+        for (int oy = ly; oy <= uy; ++oy) {
+            int cy = absy ? oy : ky + oy;
+            if (!absy) {
+                if (T.per[1]) cy -= floordiv(cy, T.n[1]) * T.n[1];
+                else if (cy < 0 || cy >= T.n[1]) continue;
+            }
+            const int row = base + T.n[0] * (cy + T.n[1] * cz);
+            const int x0 = max(kx - w[0], 0), x1 = min(kx + w[0], T.n[0] - 1);
+            for (int cx = x0; cx <= x1; ++cx) wm = fmax(wm, cellh[row + cx]);
+        }
+    }
+    reach[c] = (hc + wm) * (1.0 + kRadiusMargin);
+}
+
+template <typename R, bool kStats>
+struct DensPair {
+    const double* __restrict__ M;
+    const double* __restrict__ H;
+    const int* __restrict__ ID;
+    double hi;
+    Acc<R> acc;
+    int cnt = 0;
+    uint64_t hash = 0;
+    __device__ __forceinline__ void pair(int j, double, double, double, double d2, int sy, int sz) {
+        const R r = sqrt_r<R>((R)d2);
+        const R hh = (R)(0.5 * (hi + H[j]));
+        const R ih = rcp<R>(hh);
+        const R q = r * ih;
+        const R w = (R)kInvPi * (ih * ih * ih) * spline_w<R>(q);
+        acc.add((R)M[j] * w);
+        if (kStats) {
+            ++cnt;
+            hash += pair_tag(ID[j], sy, sz);
+        }
+    }
+};
+
+template <typename R, bool kStats>
+__global__ __launch_bounds__(kBS) void k_density(Map mp, const TubeConst* __restrict__ tc,
+                                                 const TubeDyn* __restrict__ td, const int* __restrict__ cur,
+                                                 PBuf B, const int* __restrict__ start,
+                                                 const int* __restrict__ wkey, const double* __restrict__ reach,
+                                                 double* __restrict__ pr2, double* __restrict__ csav,
+                                                 int* __restrict__ ncount, unsigned long long* __restrict__ nhash) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const TubeDyn& D = td[ip.sg.tube];
+    if (D.done || D.err) return;
+    const TubeConst& T = tc[ip.sg.tube];
+    const int s = ip.sg.species;
+    const int q = ip.sg.off + li;
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    const double* X = B.x + wo;
+    const double* Y = B.y + wo;
+    const double* Z = B.z + wo;
+    const double* H = B.h + wo;
+    double rho = B.rho[wo + q];
+    if (B.ind[wo + q] == 0) {
+        const int cell_off = T.cell_off[s];
+        const int lc = wkey[q] - cell_off;
+        const int kx = lc % T.n[0];
+        const int ky = (lc / T.n[0]) % T.n[1];
+        const int kz = lc / (T.n[0] * T.n[1]);
+        DensPair<R, kStats> f{B.m + wo, H, B.id + wo, H[q]};
+        for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
+        rho = f.acc.get();
+        B.rho[wo + q] = rho;
+        if (s == 0) {
+            const double p = (T.eos == 0) ? rho * B.e[wo + q] * (T.gamma - 1) : T.cs * T.cs * rho;
+            B.p[wo + q] = p;
+        } else {
+            B.ts[wo + q] = rho / T.K[s - 1];
+        }
+        if (kStats) {
+            ncount[q] = f.cnt;
+            nhash[q] = f.hash;
+        }
+    } else if (kStats) {
+        ncount[q] = 0;
+        nhash[q] = 0ull;
+    }
+    if (s == 0) {
+        const double p = B.p[wo + q];
+        pr2[q] = p / (rho * rho);
+        csav[q] = sqrt(T.gamma * p / rho);
+    }
+}
+
+template <typename R>
+struct ForcePair {
+    const double* __restrict__ M;
+    const double* __restrict__ H;
+    const double* __restrict__ VX;
+    const double* __restrict__ VY;
+    const double* __restrict__ VZ;
+    const double* __restrict__ RHO;
+    const double* __restrict__ PR2;
+    const double* __restrict__ CS;
+    double hi, vxi, vyi, vzi;
+    R rhoi, pr2i, csi, mi, alpha, beta, eps;
+    Acc<R> ax, ay, az, ed;
+    __device__ __forceinline__ void pair(int j, double dx, double dy, double dz, double d2, int, int) {
+        if (!(d2 > 0.0)) return;
+        const R dist = sqrt_r<R>((R)d2);
+        const R hh = (R)(0.5 * (hi + H[j]));
+        const R ih = rcp<R>(hh);
+        const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
+        // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
+        const R nu = (R)(vxi - VX[j]) * (-fx) + (R)(vyi - VY[j]) * (-fy) + (R)(vzi - VZ[j]) * (-fz);
+        R fnu = R(0);
+        if (nu < R(0)) {
+            const R cnu = R(0.5) * (csi + (R)CS[j]);
+            const R mu = hh * nu * rcp<R>(dist * dist + eps * hh * hh);
+            fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<R>(R(0.5) * (rhoi + (R)RHO[j]));
+        }
+        const R id4 = ih * ih * ih * ih;
+        const R dw = (R)kInvPi * id4 * spline_dw<R>(dist * ih);
+        const R dwr = dw * rcp<R>(dist);
+        ed.add((mi * pr2i + mi * fnu * R(0.5)) * nu * dwr);
+        const R coef = (R)M[j] * (pr2i + (R)PR2[j] + fnu) * dwr;
+        ax.add(fx * coef);
+        ay.add(fy * coef);
+        az.add(fz * coef);
+    }
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restrict__ tc,
+                                               const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+                                               const int* __restrict__ start, const int* __restrict__ wkey,
+                                               const double* __restrict__ reach, const double* __restrict__ pr2,
+                                               const double* __restrict__ csav) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const TubeDyn& D = td[ip.sg.tube];
+    if (D.done || D.err) return;
+    const TubeConst& T = tc[ip.sg.tube];
+    const int q = ip.sg.off + li;
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    if (B.ind[wo + q] != 0) {
+        B.ax[wo + q] = 0.0;
+        B.ay[wo + q] = 0.0;
+        B.az[wo + q] = 0.0;
+        B.ed[wo + q] = 0.0;
+        return;
+    }
+    const double* X = B.x + wo;
+    const double* Y = B.y + wo;
+    const double* Z = B.z + wo;
+    const double* H = B.h + wo;
+    const int cell_off = T.cell_off[0];
+    const int lc = wkey[q] - cell_off;
+    const int kx = lc % T.n[0];
+    const int ky = (lc / T.n[0]) % T.n[1];
+    const int kz = lc / (T.n[0] * T.n[1]);
+    ForcePair<R> f{B.m + wo, H, B.vx + wo, B.vy + wo, B.vz + wo, B.rho + wo, pr2, csav,
+                   H[q], B.vx[wo + q], B.vy[wo + q], B.vz[wo + q],
+                   (R)B.rho[wo + q], (R)pr2[q], (R)csav[q], (R)B.m[wo + q],
+                   (R)T.alpha, (R)T.beta, (R)T.eps};
+    for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
+    B.ax[wo + q] = f.ax.get();
+    B.ay[wo + q] = f.ay.get();
+    B.az[wo + q] = f.az.get();
+    B.ed[wo + q] = f.ed.get();
+}
+
+// kernel.cu:2310-2437 for one component; n_dust 1 or 2, reference operation order.
+__device__ __forceinline__ void idic_solve(int nd, double vg, const double* vd, double psi, const double* tb,
+                                           const double* ep, double tau, double* vgh, double* vdh) {
+    if (nd == 1) {
+        double y = vg;
+        const double x0 = vg - vd[0];
+        y = y + ep[0] * vd[0];
+        const double b0 = (tb[0] + tau) / (ep[0] * tau);
+        double beta = 1.0;
+        beta = beta + 1.0 / b0;
+        const double yh = y + tau * psi;
+        double xh0 = -1.0 * b0 * beta * (x0 + tau * psi) / (b0 * b0);
+        xh0 = xh0 + (x0 + tau * psi) / (b0 * b0);
+        xh0 = xh0 * (-1.0 * tb[0]) / (tau * ep[0] * beta);
+        double S = 1.0;
+        S = S + ep[0];
+        double g = yh;
+        g = g + ep[0] * xh0;
+        double d0 = yh;
+        d0 = d0 - (S - ep[0]) * xh0;
+        d0 = d0 / S;
+        g = g / S;
+        *vgh = g;
+        vdh[0] = d0;
+        return;
+    }
+    double y = vg;
+    const double x0 = vg - vd[0];
+    const double x1 = vg - vd[1];
+    y = y + ep[0] * vd[0] + ep[1] * vd[1];
+    const double b0 = (tb[0] + tau) / (ep[0] * tau);
+    const double b1 = (tb[1] + tau) / (ep[1] * tau);
+    double beta = 1.0;
+    beta = beta + 1.0 / b0 + 1.0 / b1;
+    const double yh = y + tau * psi;
+    double xh0 = -1.0 * b0 * beta * (x0 + tau * psi) / (b0 * b0);
+    xh0 = xh0 + (x0 + tau * psi) / (b0 * b0) + (x1 + tau * psi) / (b0 * b1);
+    xh0 = xh0 * (-1.0 * tb[0]) / (tau * ep[0] * beta);
+    double xh1 = -1.0 * b1 * beta * (x1 + tau * psi) / (b1 * b1);
+    xh1 = xh1 + (x0 + tau * psi) / (b1 * b0) + (x1 + tau * psi) / (b1 * b1);
+    xh1 = xh1 * (-1.0 * tb[1]) / (tau * ep[1] * beta);
+    double S = 1.0;
+    S = S + ep[0] + ep[1];
+    double g = yh;
+    g = g + ep[0] * xh0;
+    double d0 = yh;
+    d0 = d0 + ep[1] * xh1;
+    d0 = d0 - (S - ep[0]) * xh0;
+    d0 = d0 / S;
+    g = g + ep[1] * xh1;
+    double d1 = yh;
+    d1 = d1 + ep[0] * xh0;
+    d1 = d1 - (S - ep[1]) * xh1;
+    d1 = d1 / S;
+    g = g / S;
+    *vgh = g;
+    vdh[0] = d0;
+    vdh[1] = d1;
+}
+
+// One wave per averaging cell: sums over its f^3 neighbour cells for every species,
+// implicit drag solve, then the per-particle kick (FP64), drift, energy and h update,
+// and the cell's conservation / CFL partials.
+__global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restrict__ tc,
+                                              const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+                                              const int* __restrict__ start, double* __restrict__ part) {
+    const ItemPos ip = locate(mp, kBS / 64);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int la = ip.li + wave;
+    if (la >= ip.sg.n) return;
+    const int t = ip.sg.tube;
+    const TubeDyn& D = td[t];
+    if (D.done || D.err) return;
+    const TubeConst& T = tc[t];
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    const int f = T.f, nx = T.n[0], ny = T.n[1];
+    const int ax_ = la % T.na[0];
+    const int ay_ = (la / T.na[0]) % T.na[1];
+    const int az_ = la / (T.na[0] * T.na[1]);
+    const int nd = T.nsp - 1;
+    const double tau = D.dt;
+
+    // sums: gas count, v (3), A (3); dust count, v (3), t_stop
+    int ng = 0;
+    double sg[6] = {0, 0, 0, 0, 0, 0};
+    int ndc[kMaxDust] = {0, 0};
+    double sd[kMaxDust][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    for (int s = 0; s < T.nsp; ++s) {
+        for (int r = 0; r < f * f; ++r) {
+            const int cy = ay_ * f + r % f, cz = az_ * f + r / f;
+            const int c0 = T.cell_off[s] + ax_ * f + nx * (cy + ny * cz);
+            const int lo = start[c0], hi = start[c0 + f];
+            for (int k = lo + lane; k < hi; k += 64) {
+                if (s == 0) {
+                    ++ng;
+                    sg[0] += B.vx[wo + k];
+                    sg[1] += B.vy[wo + k];
+                    sg[2] += B.vz[wo + k];
+                    sg[3] += B.ax[wo + k];
+                    sg[4] += B.ay[wo + k];
+                    sg[5] += B.az[wo + k];
+                } else {
+                    ndc[s - 1]++;
+                    sd[s - 1][0] += B.vx[wo + k];
+                    sd[s - 1][1] += B.vy[wo + k];
+                    sd[s - 1][2] += B.vz[wo + k];
+                    sd[s - 1][3] += B.ts[wo + k];
+                }
+            }
+        }
+    }
+    ng = wave_sum_i(ng);
+    for (int a = 0; a < 6; ++a) sg[a] = wave_sum(sg[a]);
+    for (int k = 0; k < nd; ++k) {
+        ndc[k] = wave_sum_i(ndc[k]);
+        for (int a = 0; a < 4; ++a) sd[k][a] = wave_sum(sd[k][a]);
+    }
+    // means and fallbacks (kernel.cu:2126-2211)
+    double vg[3], psi[3], vdm[kMaxDust][3], tb[kMaxDust], ep[kMaxDust];
+    for (int a = 0; a < 3; ++a) {
+        vg[a] = ng > 0 ? sg[a] / ng : sg[a];
+        psi[a] = ng > 0 ? sg[3 + a] / ng : sg[3 + a];
+    }
+    for (int k = 0; k < nd; ++k) {
+        for (int a = 0; a < 3; ++a) vdm[k][a] = ndc[k] > 0 ? sd[k][a] / ndc[k] : sd[k][a];
+        tb[k] = ndc[k] > 0 ? sd[k][3] / ndc[k] : T.t_fb;
+    }
+    for (int k = 0; k < nd; ++k) {
+        const int test = (k == 1 && T.eps2_quirk) ? ndc[0] : ndc[k];
+        ep[k] = (ng > 0 && test > 0) ? ndc[k] * T.mu[k] / ng : T.eps_fb[k];
+    }
+    double vgh[3], vdh[kMaxDust][3];
+    for (int a = 0; a < 3; ++a) {
+        double vdc[kMaxDust] = {vdm[0][a], nd > 1 ? vdm[1][a] : 0.0};
+        double out[kMaxDust] = {0.0, 0.0};
+        idic_solve(nd, vg[a], vdc, psi[a], tb, ep, tau, &vgh[a], out);
+        for (int k = 0; k < nd; ++k) vdh[k][a] = out[k];
+    }
+    // gas velocity terms (IDIC.h:301-309)
+    double tmp1 = 0.0, tmpv[3] = {0.0, 0.0, 0.0};
+    if (nd == 1) {
+        tmp1 = tmp1 + ep[0] / tb[0];
+        for (int a = 0; a < 3; ++a) tmpv[a] = tmpv[a] + ep[0] * vdh[0][a] / tb[0];
+    } else {
+        tmp1 = tmp1 + ep[0] / tb[0] + ep[1] / tb[1];
+        for (int a = 0; a < 3; ++a) tmpv[a] = tmpv[a] + ep[0] * vdh[0][a] / tb[0] + ep[1] * vdh[1][a] / tb[1];
+    }
+    // per-particle update
+    double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
+    for (int s = 0; s < T.nsp; ++s) {
+        for (int r = 0; r < f * f; ++r) {
+            const int cy = ay_ * f + r % f, cz = az_ * f + r / f;
+            const int c0 = T.cell_off[s] + ax_ * f + nx * (cy + ny * cz);
+            const int lo = start[c0], hi = start[c0 + f];
+            for (int k = lo + lane; k < hi; k += 64) {
+                const long long g = wo + k;
+                double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
+                const double m = B.m[g];
+                double e = 0.0;
+                if (B.ind[g] == 0) {
+                    if (s == 0) {
+                        vx = (vx / tau + tmpv[0] + psi[0]) / (1.0 / tau + tmp1);
+                        vy = (vy / tau + tmpv[1] + psi[1]) / (1.0 / tau + tmp1);
+                        vz = (vz / tau + tmpv[2] + psi[2]) / (1.0 / tau + tmp1);
+                    } else {
+                        const double tk = tb[s - 1];
+                        vx = (vx / tau + vgh[0] / tk) / (1.0 / tau + 1.0 / tk);
+                        vy = (vy / tau + vgh[1] / tk) / (1.0 / tau + 1.0 / tk);
+                        vz = (vz / tau + vgh[2] / tk) / (1.0 / tau + 1.0 / tk);
+                    }
+                    B.vx[g] = vx;
+                    B.vy[g] = vy;
+                    B.vz[g] = vz;
+                    double x = B.x[g] + vx * tau;
+                    double y = B.y[g] + vy * tau;
+                    double z = B.z[g] + vz * tau;
+                    if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
+                    if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
+                    B.x[g] = x;
+                    B.y[g] = y;
+                    B.z[g] = z;
+                    if (s == 0) {
+                        e = B.e[g] + B.ed[g] * tau;
+                        B.e[g] = e;
+                    }
+                    double h = B.h[g];
+                    if (T.h_mode == 1) {
+                        h = T.eta * cbrt(m / B.rho[g]);
+                        B.h[g] = h;
+                    }
+                    const double v = sqrt(vx * vx + vy * vy + vz * vz);
+                    mn = fmin(mn, h / (T.cs + v));
+                } else if (s == 0) {
+                    e = B.e[g];
+                }
+                px += m * vx;
+                py += m * vy;
+                pz += m * vz;
+                double kin = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+                if (s == 0) kin += m * e;
+                E += kin;
+            }
+        }
+    }
+    px = wave_sum(px);
+    py = wave_sum(py);
+    pz = wave_sum(pz);
+    E = wave_sum(E);
+    mn = wave_min(mn);
+    if (lane == 0) {
+        double* o = part + 5ll * (T.avg_off + la);
+        o[0] = px;
+        o[1] = py;
+        o[2] = pz;
+        o[3] = E;
+        o[4] = mn;
+    }
+}
+
+// One block per tube: deterministic reduction of the averaging-cell partials, then time,
+// next step size and completion (the loop control of kernel.cu:1929-1932,2566 plus CFL).
+__global__ __launch_bounds__(kBS) void k_finalize(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                  const int* __restrict__ cur, const double* __restrict__ part) {
+    __shared__ double sh[5][kBS / 64];
+    __shared__ int skip;
+    const int t = blockIdx.x;
+    const TubeConst& T = tc[t];
+    TubeDyn& D = td[t];
+    if (threadIdx.x == 0) skip = D.done;
+    __syncthreads();
+    if (skip) return;
+    const bool failed = D.err != 0;
+    double a[4] = {0, 0, 0, 0}, mn = 1e300;
+    if (!failed) {
+        for (int i = threadIdx.x; i < T.navg; i += kBS) {
+            const double* o = part + 5ll * (T.avg_off + i);
+            for (int c = 0; c < 4; ++c) a[c] += o[c];
+            mn = fmin(mn, o[4]);
+        }
+    }
+    for (int c = 0; c < 4; ++c) a[c] = wave_sum(a[c]);
+    mn = wave_min(mn);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int c = 0; c < 4; ++c) sh[c][w] = a[c];
+        sh[4][w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int next_cur = 1 - *cur;
+    if (failed) {
+        D.done = 1;
+        D.done_cur = next_cur;
+        return;
+    }
+    double s[4] = {0, 0, 0, 0}, m2 = 1e300;
+    for (int k = 0; k < kBS / 64; ++k) {
+        for (int c = 0; c < 4; ++c) s[c] += sh[c][k];
+        m2 = fmin(m2, sh[4][k]);
+    }
+    for (int c = 0; c < 4; ++c) D.cons[c] = s[c];
+    if (!(isfinite(s[0]) && isfinite(s[1]) && isfinite(s[2]) && isfinite(s[3]))) {
+        D.err |= kErrNonFinite;
+        D.done = 1;
+        D.done_cur = next_cur;
+        return;
+    }
+    D.t = D.t + D.dt;
+    D.steps += 1;
+    int done = 0;
+    if (T.max_steps > 0 && D.steps >= T.max_steps) done = 1;
+    if (T.dt_mode == 2) {
+        if (D.final_step) {
+            done = 1;
+        } else {
+            const double rem = T.t_end - D.t;
+            if (!(rem > 0.0)) {
+                done = 1;
+            } else {
+                const double dtc = T.cfl * m2;
+                if (dtc >= rem) {
+                    D.dt = rem;
+                    D.final_step = 1;
+                } else {
+                    D.dt = dtc;
+                }
+            }
+        }
+    } else {
+        if (D.t >= T.t_end) done = 1;
+    }
+    if (done) {
+        D.done = 1;
+        D.done_cur = next_cur;
+    }
+}
+
+__global__ void k_flip(int* cur) { *cur = 1 - *cur; }
+
+inline dim3 grid1(int n) { return dim3((unsigned)((n + kBS - 1) / kBS)); }
+
+}  // namespace
+
+int scan_blocks(int n) { return (n + kScanTile - 1) / kScanTile; }
+
+hipError_t launch_prepare(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.B);
+    return hipGetLastError();
+}
+
+hipError_t launch_begin(const StepArgs& a, hipStream_t s) {
+    const int n = a.ncells > a.ntubes ? a.ncells : a.ntubes;
+    hipLaunchKernelGGL(k_begin, grid1(n), dim3(kBS), 0, s, a.count, a.ncells, a.td, a.ntubes);
+    return hipGetLastError();
+}
+
+hipError_t launch_key(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_key, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.cur, a.B, a.key, a.slot,
+                       a.count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_reduce, dim3(a.scan_nb), dim3(kBS), 0, s, a.count, a.ncells, a.scan_bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBS), 0, s, a.scan_bsum, a.scan_nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(a.scan_nb), dim3(kBS), 0, s, a.count, a.ncells, a.scan_bsum, a.start);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.key, a.slot,
+                       a.start, a.tsrc, a.tid, a.tkey);
+    return hipGetLastError();
+}
+
+hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.start, a.tsrc,
+                       a.tid, a.tkey, a.wkey);
+    return hipGetLastError();
+}
+
+hipError_t launch_cellmax(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_cellmax, dim3(a.cmap.nblocks), dim3(kBS), 0, s, a.cmap, a.cur, a.B, a.start, a.cellh,
+                       a.td);
+    return hipGetLastError();
+}
+
+hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_radius, dim3(a.cmap.nblocks), dim3(kBS), 0, s, a.cmap, a.tc, a.td, a.cellh, a.reach);
+    return hipGetLastError();
+}
+
+hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s) {
+    const dim3 g(a.pmap.nblocks), b(kBS);
+    if (precision == 0) {
+        if (stats)
+            hipLaunchKernelGGL((k_density<double, true>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey,
+                               a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+        else
+            hipLaunchKernelGGL((k_density<double, false>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start,
+                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+    } else {
+        if (stats)
+            hipLaunchKernelGGL((k_density<float, true>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey,
+                               a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+        else
+            hipLaunchKernelGGL((k_density<float, false>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start,
+                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
+    if (a.gmap.nblocks == 0) return hipSuccess;
+    const dim3 g(a.gmap.nblocks), b(kBS);
+    if (precision == 0)
+        hipLaunchKernelGGL((k_force<double>), g, b, 0, s, a.gmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey, a.reach,
+                           a.pr2, a.csav);
+    else
+        hipLaunchKernelGGL((k_force<float>), g, b, 0, s, a.gmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey, a.reach,
+                           a.pr2, a.csav);
+    return hipGetLastError();
+}
+
+hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
+                       a.part);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.cur, a.part);
+    return hipGetLastError();
+}
+
+hipError_t launch_flip(const StepArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, s, a.cur);
+    return hipGetLastError();
+}
+
+}  // namespace dsh

@@ -1,0 +1,161 @@
+"""Python host mirror of the reference's per-step protocol over the C ABI.
+
+The reference drives its step from main() over global data.h arrays
+(kernel.cu:1793-2566).  ``Simulation`` exposes the same operations: create the
+device state, upload each species in the data.h layout, advance the time loop,
+download the arrays back in original particle order, read the conservation sums.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import DshError, Params, TubeSpec, empty_arrays, make_view
+
+
+def _check(rc: int, msg: str = ""):
+    if rc != abi.OK:
+        raise DshError(rc, msg)
+
+
+def default_params() -> Params:
+    p = Params()
+    _check(abi.lib().dsh_default_params(C.byref(p)), "default_params")
+    return p
+
+
+def tube_params(spec: TubeSpec) -> Params:
+    p = Params()
+    _check(abi.lib().dsh_tube_params(C.byref(spec), C.byref(p)), "tube_params")
+    return p
+
+
+def tube_count(spec: TubeSpec) -> int:
+    n = (C.c_int * 3)()
+    _check(abi.lib().dsh_tube_count(C.byref(spec), n), "tube_count")
+    return int(n[0])
+
+
+def tube_ic(spec: TubeSpec, species: int) -> dict:
+    """Initial data.h arrays of one species of a shock tube (original particle order)."""
+    n = tube_count(spec)
+    arrs = empty_arrays(n)
+    v = make_view(arrs)
+    _check(abi.lib().dsh_tube_ic(C.byref(spec), species, C.byref(v)), "tube_ic")
+    return arrs
+
+
+def read_init(path: str) -> Params:
+    p = Params()
+    _check(abi.lib().dsh_read_init(path.encode(), C.byref(p)), f"read_init {path}")
+    return p
+
+
+UPLOAD_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "m", "rho", "p", "e", "h", "t_stop", "ind")
+
+
+class Simulation:
+    """One tube on the GPU (a batch of one in the engine)."""
+
+    def __init__(self, params: Params):
+        self._lib = abi.lib()
+        self.params = params
+        self._ctx = C.c_void_p()
+        _check(self._lib.dsh_create(C.byref(params), C.byref(self._ctx)), "dsh_create")
+        self.counts = {}
+
+    @classmethod
+    def from_spec(cls, spec: TubeSpec) -> "Simulation":
+        sim = cls(tube_params(spec))
+        for s in (0, 1):
+            sim.upload(s, tube_ic(spec, s))
+        return sim
+
+    def _err(self) -> str:
+        m = self._lib.dsh_last_error(self._ctx)
+        return m.decode() if m else ""
+
+    def _call(self, rc: int):
+        if rc != abi.OK:
+            raise DshError(rc, self._err())
+
+    def upload(self, species: int, arrays: dict):
+        n = len(arrays["x"])
+        keep = {f: np.ascontiguousarray(arrays[f], dtype=np.int32 if f == "ind" else np.float64)
+                for f in UPLOAD_FIELDS if f in arrays and arrays[f] is not None}
+        v = make_view(keep)
+        self._call(self._lib.dsh_upload_species(self._ctx, species, C.byref(v), n - 1))
+        self.counts[species] = n
+
+    def step(self, nsteps: int = 1) -> float:
+        t = C.c_double(0.0)
+        self._call(self._lib.dsh_step(self._ctx, nsteps, C.byref(t)))
+        return t.value
+
+    def download(self, species: int, fields=abi.ALL_FIELDS) -> dict:
+        arrs = empty_arrays(self.counts[species], fields)
+        v = make_view(arrs)
+        self._call(self._lib.dsh_download_species(self._ctx, species, C.byref(v)))
+        return arrs
+
+    def conservation(self) -> np.ndarray:
+        out = np.zeros(4)
+        self._call(self._lib.dsh_conservation(self._ctx, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def status(self) -> dict:
+        t, dt = C.c_double(), C.c_double()
+        steps, done = C.c_int(), C.c_int()
+        self._call(self._lib.dsh_status(self._ctx, C.byref(t), C.byref(dt), C.byref(steps), C.byref(done)))
+        return {"t": t.value, "dt": dt.value, "steps": steps.value, "done": bool(done.value)}
+
+    def grid(self) -> dict:
+        d = np.zeros(9)
+        i = np.zeros(5, np.int32)
+        self._call(self._lib.dsh_grid(self._ctx, d.ctypes.data_as(C.POINTER(C.c_double)),
+                                      i.ctypes.data_as(C.POINTER(C.c_int))))
+        return {"origin": d[0:3], "cell": d[3:6], "length": d[6:9], "n": i[0:3], "f": int(i[3]),
+                "n_avg": int(i[4])}
+
+    def set_stats(self, on: bool = True):
+        self._call(self._lib.dsh_set_stats(self._ctx, int(on)))
+
+    def neighbour_stats(self, species: int):
+        n = self.counts[species]
+        cnt = np.zeros(n, np.int32)
+        h = np.zeros(n, np.uint64)
+        self._call(self._lib.dsh_neighbour_stats(self._ctx, species, cnt.ctypes.data_as(C.POINTER(C.c_int)),
+                                                 h.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return cnt, h
+
+    def set_timing(self, on: bool = True):
+        self._call(self._lib.dsh_set_timing(self._ctx, int(on)))
+
+    def kernel_times(self) -> dict:
+        nmax, ln = 32, 32
+        ms = np.zeros(nmax)
+        names = C.create_string_buffer(nmax * ln)
+        n = self._lib.dsh_kernel_times(self._ctx, ms.ctypes.data_as(C.POINTER(C.c_double)), names, nmax, ln)
+        if n < 0:
+            raise DshError(n, self._err())
+        raw = names.raw
+        return {raw[i * ln:(i + 1) * ln].split(b"\0")[0].decode(): float(ms[i]) for i in range(min(n, nmax))}
+
+    def close(self):
+        if self._ctx:
+            self._lib.dsh_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,164 @@
+/*
+ * dsh.h - C ABI of the MI355X-native dusty-gas SPH step (gas + dust, IDIC drag).
+ *
+ * The reference (DustyShock3D) has no library API: its operator boundary is the
+ * set of kernel signatures plus the launch protocol inside main() (kernel.cu:1929-2566,
+ * IDIC.h:2-346), over the global structure-of-arrays state of data.h.  Every entry
+ * point below replaces one part of that protocol; the comment on each names the
+ * reference lines it stands in for.
+ *
+ * Conventions
+ *   - plain C types only; every function returns an int status (DSH_OK = 0,
+ *     negative on error) and never aborts; dsh_last_error() explains the last failure.
+ *   - particle arrays are in the data.h layout: one double (or int) array per field,
+ *     indexed by the original particle index 0..Pm (Pm = count - 1, kernel.cu:255).
+ *     Uploads borrow the caller's arrays for the duration of the call only.
+ *   - species 0 is gas, species 1..n_dust are dust species (data.h:19-20,49-55).
+ *   - Ind / ind: 0 = active particle, 1 = frozen wall particle (IDIC.h:8).
+ *   - a context owns all of its device memory; one context per host thread.
+ */
+#ifndef DSH_H_
+#define DSH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSH_MAX_DUST 2
+#define DSH_MAX_SPECIES (1 + DSH_MAX_DUST)
+
+#define DSH_OK 0
+#define DSH_ERR_ARG (-1)
+#define DSH_ERR_HIP (-2)
+#define DSH_ERR_NOMEM (-3)
+#define DSH_ERR_STATE (-4)
+#define DSH_ERR_GRID (-5)        /* smoothing lengths outgrew the neighbour grid */
+#define DSH_ERR_NONFINITE (-6)   /* a tube produced NaN/Inf */
+#define DSH_ERR_IO (-7)
+#define DSH_ERR_BUDGET (-8)      /* request larger than the HBM budget */
+#define DSH_ERR_NOTFOUND (-9)
+#define DSH_ERR_UNSUPPORTED (-10)
+
+/* dt_mode */
+#define DSH_DT_FIXED 0       /* tau from Init.txt every step (kernel.cu:1931) */
+#define DSH_DT_CFL_ONCE 1    /* CFL step computed from the initial state, then fixed */
+#define DSH_DT_CFL 2         /* CFL step recomputed every step, last step lands on t_end */
+
+/* precision */
+#define DSH_FP64 0           /* all arithmetic in FP64 */
+#define DSH_MIXED 1          /* pair sums in FP32 with Kahan compensation; drag, positions,
+                                conservation sums in FP64 */
+
+/*
+ * Run parameters.  The first block is Init.txt lines 2-18 in file order
+ * (parser kernel.cu:1705-1722); the second block is the extension lines that
+ * follow them in an extended Init.txt (old files simply omit them).
+ */
+typedef struct dsh_params {
+    double x_min, x_max, y_min, y_max, z_min, z_max;
+    int    eos_type;              /* 0: p = (gamma-1) rho e, 1: p = cs^2 rho (kernel.cu:92-97) */
+    int    max_particles;         /* Maximum_particle (capacity hint) */
+    double particles_per_length;  /* Particle_on_length */
+    double h;                     /* global smoothing length (h_mode 0) */
+    double avg_cell_coeff;        /* Coeff_h_dust_cell: averaging-cell width / h */
+    double tau;                   /* fixed time step (dt_mode 0) */
+    double t_end;
+    double t_out;
+    double alpha, beta, eps;      /* Monaghan viscosity (IDIC.h:225-235) */
+
+    int    periodic_y, periodic_z;
+    double gamma;                 /* Gam_g (data.h:7); also used by the viscosity sound speed */
+    double cs;                    /* Speed_of_sound_gas (kernel.cu:1744) */
+    int    h_mode;                /* 0: global h, 1: h_i = eta (m_i / rho_i)^(1/3) */
+    double eta;
+    int    n_dust;                /* 1 or 2 dust species */
+    int    drag_mode;             /* 1: t_stop = rho_dust / K (only mode implemented) */
+    double K[DSH_MAX_DUST];
+    double R_dust[DSH_MAX_DUST];  /* reserved for the reference stopping-time law (IDIC.h:78-91) */
+    double mass_ratio[DSH_MAX_DUST];   /* m_dust / m_gas: eps = n_d mu / n_g (kernel.cu:2192-2211) */
+    double eps_fallback[DSH_MAX_DUST]; /* 0.5, 0.33 in the reference (kernel.cu:2200,2209) */
+    double tstop_fallback;             /* 1.0 in the reference (kernel.cu:2160,2188) */
+    int    eps2_uses_count1;           /* reproduce the test of kernel.cu:2203 */
+    int    dt_mode;
+    double cfl;
+    int    max_steps;             /* 0: no step limit */
+    double cell_size;             /* 0: automatic, 2 h_min (1 + cell_slack) */
+    double cell_slack;
+    int    avg_factor;            /* averaging cell = avg_factor^3 neighbour cells */
+    int    precision;             /* DSH_FP64 or DSH_MIXED */
+} dsh_params;
+
+/* data.h structure-of-arrays view; NULL fields are skipped. */
+typedef struct dsh_soa_view {
+    double *x, *y, *z;
+    double *vx, *vy, *vz;
+    double *ax, *ay, *az;         /* gas acceleration of the last step (download only) */
+    double *m, *rho, *p, *e, *h;
+    double *t_stop;               /* dust stopping time */
+    double *edot;                 /* gas de/dt of the last step (download only) */
+    int    *ind;
+    int    *nn;                   /* next particle in the same cell, -1 ends (download only) */
+    int    *cell;                 /* neighbour-cell key of the last step (download only) */
+} dsh_soa_view;
+
+/* One isothermal dusty shock tube: the problem the benchmark configurations use. */
+typedef struct dsh_tube_spec {
+    double x_min, x_max;          /* -0.5, 0.5 */
+    double y_width, z_width;      /* periodic cross-section [0, w) */
+    double dx_left;               /* lattice spacing for x <= 0; x > 0 uses 2 dx_left */
+    double rho_left;              /* gas density left; right is rho_left / 8 */
+    double dust_to_gas;
+    double K;                     /* drag coefficient, t_stop = rho_dust / K */
+    double jitter;                /* uniform position noise, fraction of local spacing */
+    double frozen_h;              /* frozen x-end layers, in units of h */
+    double t_end;
+    double cfl;
+    int    dt_mode;
+    int    max_steps;
+    int    precision;
+    int    tube_index;            /* RNG stream of this tube */
+    uint64_t seed;
+} dsh_tube_spec;
+
+/* ---- problem set-up (host only; replaces init_Sod_X_Dust, kernel.cu:1391-1693) ---- */
+int dsh_default_params(dsh_params* out);
+int dsh_tube_params(const dsh_tube_spec* spec, dsh_params* out);
+int dsh_tube_count(const dsh_tube_spec* spec, int* n_per_species);
+int dsh_tube_ic(const dsh_tube_spec* spec, int species, const dsh_soa_view* out);
+/* Init.txt reader (kernel.cu:1705-1742); extension lines after line 18 are optional. */
+int dsh_read_init(const char* path, dsh_params* out);
+
+/* ---- single-tube context (replaces kernel.cu:1793-1926 set-up and 1929-2566 loop) ---- */
+typedef struct dsh_ctx dsh_ctx;
+int dsh_create(const dsh_params* params, dsh_ctx** out);
+/* Upload species `species` with pm+1 particles (H2D of kernel.cu:1878-1926). */
+int dsh_upload_species(dsh_ctx* ctx, int species, const dsh_soa_view* view, int pm);
+/* Advance up to nsteps steps (fewer if the run reaches t_end / max_steps).
+   t_inout receives the simulated time afterwards (Tm, kernel.cu:1931). */
+int dsh_step(dsh_ctx* ctx, int nsteps, double* t_inout);
+/* Fill the view in original particle order (D2H of kernel.cu:2531-2558). */
+int dsh_download_species(dsh_ctx* ctx, int species, const dsh_soa_view* view);
+/* out[0..2] = total momentum, out[3] = total energy (FP64, after the last step). */
+int dsh_conservation(dsh_ctx* ctx, double out[4]);
+int dsh_status(dsh_ctx* ctx, double* t, double* dt, int* steps, int* done);
+/* Neighbour grid built at the first step (the role of Clx..Cnz, kernel.cu:1767-1778):
+   dbl[0..2] origin, [3..5] cell size, [6..8] axis length; ints[0..2] cells per axis,
+   ints[3] averaging factor, ints[4] averaging cells in total. */
+int dsh_grid(dsh_ctx* ctx, double dbl[9], int ints[5]);
+/* Neighbour-set fingerprint of the last density pass, original particle order:
+   count = neighbours incl. self and periodic images, hash = order-free sum of ids. */
+int dsh_set_stats(dsh_ctx* ctx, int on);
+int dsh_neighbour_stats(dsh_ctx* ctx, int species, int* count, uint64_t* hash);
+/* Per-kernel GPU time of the last timed step, ms (HIP events on the step stream). */
+int dsh_set_timing(dsh_ctx* ctx, int on);
+int dsh_kernel_times(dsh_ctx* ctx, double* ms, char* names, int max_kernels, int name_len);
+const char* dsh_last_error(dsh_ctx* ctx);
+void dsh_destroy(dsh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSH_H_ */

@@ -1,0 +1,123 @@
+"""GPU (HIP through the C ABI) against the FP64 oracle.
+
+Bit-exact on a shared state: neighbour-cell keys, the sorted (cell, id) order as the
+reference's Nn linked lists (kernel.cu:1937-1995) and the neighbour sets (count and an
+order-free hash of (id, periodic image) per particle).  Fields within tolerance:
+  FP64 mode   density, acceleration, velocities ~1e-10 (summation order only)
+  mixed mode  density <= 1e-5 pointwise, velocities <= 1e-5 normwise (FP32 pair sums)
+  momentum    |dP| / max(|P|, sum m|v|) <= 1e-9
+"""
+import numpy as np
+import pytest
+
+from dustyshock3d_amd import Simulation, configs
+from dustyshock3d_amd.abi import FP64, MIXED
+from tests.parity_util import inject, momentum_error, rel_normwise, rel_pointwise, take_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_state(oracle_mod, spec, steps):
+    o = oracle_mod.OracleSim.from_spec(spec)
+    if steps:
+        o.step(steps)
+    return o.params, take_state(o)
+
+
+def _one_step_both(oracle_mod, params, state, precision):
+    params.precision = precision
+    o = inject(oracle_mod.OracleSim, params, state)
+    g = inject(Simulation, params, state)
+    g.set_stats(True)
+    o.step(1)
+    g.step(1)
+    return o, g
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("warm_steps", [0, 20])
+def test_step_parity_shared_state(oracle_mod, precision, warm_steps):
+    spec = configs.cfg0()
+    params, state = _oracle_state(oracle_mod, spec, warm_steps)
+    o, g = _one_step_both(oracle_mod, params, state, precision)
+    print(f"\ncase precision={precision} warm={warm_steps}", flush=True)
+    for s in (0, 1):
+        ro, rg = o.download(s), g.download(s)
+        # integer work: bit-exact
+        assert np.array_equal(rg["cell"], ro["cell"]), f"species {s}: cell keys differ"
+        assert np.array_equal(rg["nn"], ro["nn"]), f"species {s}: sorted order (Nn lists) differs"
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co), f"species {s}: neighbour counts differ"
+        assert np.array_equal(hg, ho), f"species {s}: neighbour sets differ"
+        act = ro["ind"] == 0
+        tol_rho = 1e-12 if precision == FP64 else 1e-5
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        tol_v = 1e-10 if precision == FP64 else 1e-5
+        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        e_x = max(float(np.max(np.abs(rg[k] - ro[k]))) for k in ("x", "y", "z"))
+        print(f"species {s}: rho {e_rho:.3e} v {e_v:.3e} x {e_x:.3e}")
+        assert e_rho <= tol_rho
+        assert e_v <= tol_v
+        assert e_x <= 1e-12
+        if s == 0:
+            tol_a = 1e-10 if precision == FP64 else 1e-5
+            e_a = max(rel_normwise(rg[k], ro[k]) for k in ("ax", "ay", "az"))
+            e_ed = rel_normwise(rg["edot"], ro["edot"])
+            print(f"gas A {e_a:.3e} edot {e_ed:.3e}")
+            assert e_a <= tol_a
+            assert e_ed <= tol_a
+        else:
+            e_ts = rel_pointwise(rg["t_stop"][act], ro["t_stop"][act])
+            assert e_ts <= tol_rho
+    st_o = take_state(o)
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    print(f"momentum {e_p:.3e}")
+    assert e_p <= 1e-9
+    so, sg = o.status(), g.status()
+    assert sg["steps"] == so["steps"] == 1
+    assert abs(sg["t"] - so["t"]) <= 1e-15
+    assert abs(sg["dt"] - so["dt"]) <= 1e-15 * so["dt"]
+
+
+def test_cfg0_run_fp64(oracle_mod):
+    """cfg0 end to end (50 fixed CFL steps, FP64) against the oracle."""
+    spec = configs.cfg0()
+    o = oracle_mod.OracleSim.from_spec(spec)
+    g = Simulation.from_spec(spec)
+    o.step(50)
+    g.step(50)
+    so, sg = o.status(), g.status()
+    assert sg["steps"] == so["steps"] == 50 and sg["done"] and so["done"]
+    st_o = take_state(o)
+    for s in (0, 1):
+        ro, rg = st_o[s], g.download(s)
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        print(f"\ncfg0 species {s}: rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= 1e-5
+        assert e_v <= 1e-5
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    print(f"momentum {e_p:.3e}")
+    assert e_p <= 1e-9
+
+
+def test_cfg0_geometry_mixed_run(oracle_mod):
+    """Mixed precision over 20 steps of the cfg0 tube: density and velocity <= 1e-5."""
+    spec = configs.cfg0(precision=MIXED, max_steps=20)
+    o = oracle_mod.OracleSim.from_spec(spec)
+    g = Simulation.from_spec(spec)
+    o.step(20)
+    g.step(20)
+    st_o = take_state(o)
+    for s in (0, 1):
+        ro, rg = st_o[s], g.download(s)
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        print(f"\nmixed species {s}: rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= 1e-5
+        assert e_v <= 1e-5
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    print(f"momentum {e_p:.3e}")
