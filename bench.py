@@ -76,6 +76,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -86,6 +87,7 @@ def main():
     spec = getattr(configs, args.config)(max_steps=0)
     t_setup = time.perf_counter()
     sim = Simulation.from_spec(spec)
+    sim.set_pair_variant(args.variant)
     n_gas, n_dust = sim.counts[0], sim.counts[1]
     sim.step(args.warmup)
     setup_s = time.perf_counter() - t_setup
@@ -139,7 +141,8 @@ def main():
         "config": {"workload": f"{args.config}: dusty shock tube, {n_gas} gas + {n_dust} dust particles, "
                                f"y,z width {spec.y_width}, dx {spec.dx_left}/{2 * spec.dx_left}, K={spec.K}, CFL 0.3",
                    "particles": n_gas + n_dust, "averaging_cells": n_avg, "t_at_start": st0["t"],
-                   "dt": st1["dt"], "setup_s": round(setup_s, 2)},
+                   "dt": st1["dt"], "setup_s": round(setup_s, 2),
+                   "pair_kernels": "tiled" if args.variant == 1 else "direct"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms},

@@ -30,6 +30,7 @@ ERR_UNSUPPORTED = -10
 
 DT_FIXED, DT_CFL_ONCE, DT_CFL = 0, 1, 2
 FP64, MIXED = 0, 1
+PAIR_DIRECT, PAIR_TILED = 0, 1
 
 _D2 = C.c_double * 2
 
@@ -150,6 +151,7 @@ def _declare(lib):
         "dsh_status": (C.c_int, [vp, _DP, _DP, _IP, _IP]),
         "dsh_grid": (C.c_int, [vp, _DP, _IP]),
         "dsh_set_stats": (C.c_int, [vp, C.c_int]),
+        "dsh_set_pair_variant": (C.c_int, [vp, C.c_int]),
         "dsh_neighbour_stats": (C.c_int, [vp, C.c_int, _IP, P(C.c_uint64)]),
         "dsh_set_timing": (C.c_int, [vp, C.c_int]),
         "dsh_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),

@@ -87,6 +87,12 @@ int dsh_grid(dsh_ctx* c, double dbl[9], int ints[5]) {
     return DSH_OK;
 }
 
+int dsh_set_pair_variant(dsh_ctx* c, int variant) {
+    if (!c || (variant != DSH_PAIR_DIRECT && variant != DSH_PAIR_TILED)) return DSH_ERR_ARG;
+    c->eng->set_pair_variant(variant);
+    return DSH_OK;
+}
+
 int dsh_set_stats(dsh_ctx* c, int on) {
     if (!c) return DSH_ERR_ARG;
     c->eng->set_stats(on != 0);

@@ -232,6 +232,10 @@ int Engine::prepare() {
     if ((rc = make_map(gseg_, 256, a_.gmap))) return rc;
     if ((rc = make_map(cseg_, 256, a_.cmap))) return rc;
     if ((rc = make_map(aseg_, 4, a_.amap))) return rc;
+    if ((rc = make_map(pseg_, 64, a_.pmap64))) return rc;
+    if ((rc = make_map(gseg_, 64, a_.gmap64))) return rc;
+    a_.variant = pair_variant_;
+    if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
 
     // particle buffers (two copies of each persistent field)
     PBuf& B = a_.B;
@@ -239,8 +243,11 @@ int Engine::prepare() {
     const size_t n2 = 2 * (size_t)ntot_;
     double** dfields[] = {&B.x, &B.y, &B.z, &B.vx, &B.vy, &B.vz, &B.m, &B.h, &B.e, &B.rho,
                           &B.p, &B.ts, &B.ax, &B.ay, &B.az, &B.ed};
-    for (double** f : dfields)
+    for (double** f : dfields) {
         if ((rc = alloc(f, n2))) return rc;
+        // fields a species does not use (gas t_stop, dust e/p/A) stay zero in both buffers
+        if ((rc = hip_check(hipMemset(*f, 0, n2 * sizeof(double)), "hipMemset"))) return rc;
+    }
     if ((rc = alloc(&B.ind, n2))) return rc;
     if ((rc = alloc(&B.id, n2))) return rc;
     int** ifields[] = {&a_.key, &a_.slot, &a_.tsrc, &a_.tid, &a_.tkey, &a_.wkey, &a_.ncount};

@@ -49,6 +49,12 @@ public:
     void set_stats(bool on) { stats_ = on; graph_dirty_ = true; }
     void set_timing(bool on) { timing_ = on; }
     void set_graph(bool on) { use_graph_ = on; }
+    void set_pair_variant(int v) {
+        pair_variant_ = v;
+        a_.variant = v;
+        graph_dirty_ = true;
+    }
+    int pair_variant() const { return pair_variant_; }
     // per-stage GPU time (ms) averaged over the steps launched with timing on
     const std::vector<std::pair<std::string, double>>& kernel_times() const { return ktimes_; }
     int timed_steps() const { return timed_steps_; }
@@ -75,6 +81,7 @@ private:
     hipGraphExec_t graph_exec_ = nullptr;
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
     int precision_ = 0;
+    int pair_variant_ = kPairTiled;
     long long ntot_ = 0;
     size_t dev_bytes_ = 0;
     std::vector<std::pair<std::string, double>> ktimes_;

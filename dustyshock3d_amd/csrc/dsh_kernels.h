@@ -15,6 +15,11 @@ struct Map {
     int nblocks;
 };
 
+// pair-kernel variants: one thread per target with global-memory candidates, or one
+// wave per 64-target chunk with LDS-staged candidate rows
+constexpr int kPairDirect = 0;
+constexpr int kPairTiled = 1;
+
 struct StepArgs {
     const TubeConst* tc;
     TubeDyn* td;
@@ -37,6 +42,9 @@ struct StepArgs {
     double* part;          // 5 per averaging cell
     // launch maps
     Map pmap, gmap, cmap, amap;
+    Map pmap64, gmap64;    // 64 targets per block (tiled pair kernels)
+    unsigned long long* masks;   // gas density hit masks reused by the force pass
+    int variant;
 };
 
 hipError_t launch_prepare(const StepArgs& a, hipStream_t s);
@@ -54,5 +62,6 @@ hipError_t launch_finalize(const StepArgs& a, hipStream_t s);
 hipError_t launch_flip(const StepArgs& a, hipStream_t s);
 
 int scan_blocks(int n);
+size_t mask_words(int gas_chunks);
 
 }  // namespace dsh

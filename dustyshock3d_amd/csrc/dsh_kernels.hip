@@ -660,6 +660,307 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
     B.ed[wo + q] = f.ed.get();
 }
 
+// ---------------------------------------------------------------- tiled pair kernels
+// One wave per chunk of 64 consecutive sorted targets.  Targets of one x-row of cells
+// are handled together: for every (dy,dz) row offset the union of their candidate
+// cells is one contiguous sorted range, staged into LDS; each lane then runs the FP64
+// predicate over its own cell range only, collecting hits in a 64-bit mask per window of
+// 64 candidates, and does the pair arithmetic for the hits alone.  The candidate set of
+// a lane is the one of the direct kernels (cells within ceil(reach / c_a) of its own),
+// so the neighbour sets are identical.
+constexpr int kTileP = 192;     // candidates staged per piece
+constexpr int kMaskSlots = 48;  // stored hit-mask words per lane of a gas chunk
+
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+    // consecutive workgroups are dealt round-robin to the 8 XCDs; give each XCD a
+    // contiguous run of chunks so neighbouring chunks share that XCD's L2
+    const int per = nb >> 3;
+    if (b >= per * 8) return b;
+    return (b & 7) * per + (b >> 3);
+}
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <typename R, int KIND, bool kStats>
+struct TileLds {
+    double x[kTileP], y[kTileP], z[kTileP], h[kTileP];
+    double vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
+    R m[kTileP];
+    R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
+    int ind[KIND ? kTileP : 1];
+    int id[kStats ? kTileP : 1];
+};
+
+// Pressure + viscosity terms of one pair (IDIC.h:225-250): returns the acceleration
+// coefficient of (dx,dy,dz) and the de/dt contribution.
+template <typename T>
+__device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
+                                            T pr2i, T pr2j, T csi, T csj, T mi, T mj, T alpha, T beta, T epsv,
+                                            T& coef, T& edc) {
+    const T ih = rcp<T>(hh);
+    // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
+    const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
+    T fnu = T(0);
+    if (nu < T(0)) {
+        const T cnu = T(0.5) * (csi + csj);
+        const T mu = hh * nu * rcp<T>(dist * dist + epsv * hh * hh);
+        fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
+    }
+    const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw<T>(dist * ih);
+    const T dwr = dw * rcp<T>(dist);
+    edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
+    coef = mj * (pr2i + pr2j + fnu) * dwr;
+}
+
+// KIND 0: density (all species); KIND 1: gas force and de/dt.
+// The gas density pass stores its predicate hit masks (one 64-bit word per lane per
+// uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
+// the same iteration order and reads them instead of re-running the predicate.
+template <typename R, int KIND, bool kStats>
+__global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
+                                                    const TubeDyn* __restrict__ td, const int* __restrict__ cur,
+                                                    PBuf B, const int* __restrict__ start,
+                                                    const int* __restrict__ wkey, const double* __restrict__ reach,
+                                                    double* __restrict__ pr2, double* __restrict__ csav,
+                                                    int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
+                                                    unsigned long long* __restrict__ masks, int mask_chunks) {
+    __shared__ TileLds<R, KIND, kStats> L;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int sgi = find_seg(mp, b);
+    const Seg sg = mp.seg[sgi];
+    const int lane = threadIdx.x;
+    const int li = (b - mp.blk[sgi]) * 64 + lane;
+    const TubeDyn& D = td[sg.tube];
+    if (D.done || D.err) return;
+    const TubeConst& T = tc[sg.tube];
+    const int s = sg.species;
+    const bool valid = li < sg.n;
+    const int q = sg.off + (valid ? li : sg.n - 1);
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    const int nx = T.n[0], ny = T.n[1], nz = T.n[2];
+    const int cell_off = T.cell_off[s];
+    const bool active = valid && B.ind[wo + q] == 0;
+    const int lc = wkey[q] - cell_off;
+    const int kx = lc % nx;
+    const int row = lc / nx;
+    const double rch = reach[wkey[q]];
+    const double xi = B.x[wo + q], yi = B.y[wo + q], zi = B.z[wo + q], hi = B.h[wo + q];
+    const int rxl = (int)ceil(rch * T.ic[0]);
+    const int ryl = (int)ceil(rch * T.ic[1]);
+    const int rzl = (int)ceil(rch * T.ic[2]);
+    // hit masks: written by the gas density pass, read by the force pass
+    const bool use_masks = masks != nullptr && s == 0 && b < mask_chunks;
+    unsigned long long* mrow = use_masks ? masks + (size_t)b * kMaskSlots * 64 + lane : nullptr;
+    int slot = 0;
+
+    Acc<R> a0, a1, a2, a3;  // density: a0; force: ax, ay, az, de/dt
+    double d0 = 0.0, d1 = 0.0, d2s = 0.0, d3 = 0.0;  // force, FP64 terms of frozen neighbours
+    int cnt = 0;
+    uint64_t hash = 0;
+    double vxi = 0, vyi = 0, vzi = 0, rhoi_d = 1, pr2i_d = 0, csi_d = 0, mi_d = 0;
+    if (KIND == 1) {
+        vxi = B.vx[wo + q];
+        vyi = B.vy[wo + q];
+        vzi = B.vz[wo + q];
+        rhoi_d = B.rho[wo + q];
+        pr2i_d = pr2[q];
+        csi_d = csav[q];
+        mi_d = B.m[wo + q];
+    }
+    const R rhoi = (R)rhoi_d, pr2i = (R)pr2i_d, csi = (R)csi_d, mi = (R)mi_d;
+    const R alpha = (R)T.alpha, beta = (R)T.beta, epsv = (R)T.eps;
+
+    bool pending = active;
+    while (true) {
+        const unsigned long long pm = __ballot(pending);
+        if (pm == 0ull) break;
+        const int leader = __ffsll((long long)pm) - 1;
+        const int row0 = __shfl(row, leader, 64);
+        const bool mine = pending && row == row0;
+        const int cxa = wave_min_int(mine ? kx : 0x7fffffff);
+        const int cxb = wave_max_int(mine ? kx : -1);
+        const double rmax = wave_max(mine ? rch : 0.0);
+        const int Rx = (int)ceil(rmax * T.ic[0]);
+        const int Ry = (int)ceil(rmax * T.ic[1]);
+        const int Rz = (int)ceil(rmax * T.ic[2]);
+        const int ky0 = row0 % ny, kz0 = row0 / ny;
+        for (int oz = -Rz; oz <= Rz; ++oz) {
+            int cz = kz0 + oz, sz = 0;
+            if (T.per[2]) {
+                sz = floordiv(cz, nz);
+                cz -= sz * nz;
+            } else if (cz < 0 || cz >= nz) {
+                continue;
+            }
+            const double shz = (double)sz * T.L[2];
+            for (int oy = -Ry; oy <= Ry; ++oy) {
+                int cy = ky0 + oy, sy = 0;
+                if (T.per[1]) {
+                    sy = floordiv(cy, ny);
+                    cy -= sy * ny;
+                } else if (cy < 0 || cy >= ny) {
+                    continue;
+                }
+                const bool need = mine && abs(oz) <= rzl && abs(oy) <= ryl;
+                if (__ballot(need) == 0ull) continue;
+                const double shy = (double)sy * T.L[1];
+                const bool shifted = (sy != 0) || (sz != 0);
+                const int rowbase = cell_off + nx * (cy + ny * cz);
+                const int ub = start[rowbase + max(cxa - Rx, 0)];
+                const int ue = start[rowbase + min(cxb + Rx, nx - 1) + 1];
+                int jl = 0, jh = 0;
+                if (need) {
+                    jl = start[rowbase + max(kx - rxl, 0)];
+                    jh = start[rowbase + min(kx + rxl, nx - 1) + 1];
+                }
+                for (int pb = ub; pb < ue; pb += kTileP) {
+                    const int pn = min(kTileP, ue - pb);
+                    __syncthreads();
+                    for (int k = lane; k < pn; k += 64) {
+                        const long long g = wo + pb + k;
+                        L.x[k] = B.x[g];
+                        L.y[k] = B.y[g];
+                        L.z[k] = B.z[g];
+                        L.h[k] = B.h[g];
+                        L.m[k] = (R)B.m[g];
+                        if (KIND == 1) {
+                            L.vx[k] = B.vx[g];
+                            L.vy[k] = B.vy[g];
+                            L.vz[k] = B.vz[g];
+                            L.rho[k] = (R)B.rho[g];
+                            L.pr2[k] = (R)pr2[pb + k];
+                            L.cs[k] = (R)csav[pb + k];
+                            L.ind[k] = B.ind[g];
+                        }
+                        if (kStats) L.id[k] = B.id[g];
+                    }
+                    __syncthreads();
+                    const int a = max(jl, pb) - pb;
+                    const int e = min(jh, pb + pn) - pb;
+                    const int len = e > a ? e - a : 0;
+                    const int maxlen = wave_max_int(len);
+                    for (int w0 = 0; w0 < maxlen; w0 += 64, ++slot) {
+                        const int wn = min(64, maxlen - w0);
+                        unsigned long long mask = 0ull;
+                        if (KIND == 1 && use_masks && slot < kMaskSlots) {
+                            mask = mrow[(size_t)slot * 64];
+                        } else if (!shifted) {
+                            for (int t = 0; t < wn; ++t) {
+                                const int u = a + w0 + t;
+                                if (u < e) {
+                                    const double dx = L.x[u] - xi;
+                                    const double dy = L.y[u] - yi;
+                                    const double dz = L.z[u] - zi;
+                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                                    const double hs = hi + L.h[u];
+                                    if (dd < hs * hs) mask |= 1ull << t;
+                                }
+                            }
+                        } else {
+                            for (int t = 0; t < wn; ++t) {
+                                const int u = a + w0 + t;
+                                if (u < e) {
+                                    const double dx = L.x[u] - xi;
+                                    const double dy = (L.y[u] - yi) + shy;
+                                    const double dz = (L.z[u] - zi) + shz;
+                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                                    const double hs = hi + L.h[u];
+                                    if (dd < hs * hs) mask |= 1ull << t;
+                                }
+                            }
+                        }
+                        if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
+                        while (mask) {
+                            const int t = __ffsll((long long)mask) - 1;
+                            mask &= mask - 1ull;
+                            const int u = a + w0 + t;
+                            const double dx = L.x[u] - xi;
+                            const double dy = (L.y[u] - yi) + shy;
+                            const double dz = (L.z[u] - zi) + shz;
+                            const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                            if (KIND == 0) {
+                                const R r = sqrt_r<R>((R)dd);
+                                const R hh = (R)(0.5 * (hi + L.h[u]));
+                                const R ih = rcp<R>(hh);
+                                const R w = (R)kInvPi * (ih * ih * ih) * spline_w<R>(r * ih);
+                                a0.add(L.m[u] * w);
+                                if (kStats) {
+                                    ++cnt;
+                                    hash += pair_tag(L.id[u], sy, sz);
+                                }
+                            } else if (dd > 0.0) {
+                                const double hhd = 0.5 * (hi + L.h[u]);
+                                if (sizeof(R) == 4 && L.ind[u] != 0) {
+                                    // a frozen neighbour has no partner term: FP64 keeps the
+                                    // momentum it injects exact to rounding
+                                    const long long g = wo + pb + u;
+                                    double coef, edc;
+                                    force_terms<double>(sqrt(dd), hhd, dx, dy, dz, vxi - L.vx[u], vyi - L.vy[u],
+                                                        vzi - L.vz[u], rhoi_d, B.rho[g], pr2i_d, pr2[pb + u], csi_d,
+                                                        csav[pb + u], mi_d, B.m[g], T.alpha, T.beta, T.eps, coef,
+                                                        edc);
+                                    d0 += dx * coef;
+                                    d1 += dy * coef;
+                                    d2s += dz * coef;
+                                    d3 += edc;
+                                } else {
+                                    R coef, edc;
+                                    const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
+                                    force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, (R)(vxi - L.vx[u]),
+                                                   (R)(vyi - L.vy[u]), (R)(vzi - L.vz[u]), rhoi, L.rho[u], pr2i,
+                                                   L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha, beta, epsv, coef, edc);
+                                    a0.add(fx * coef);
+                                    a1.add(fy * coef);
+                                    a2.add(fz * coef);
+                                    a3.add(edc);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        pending = pending && !mine;
+    }
+
+    if (!valid) return;
+    if (KIND == 0) {
+        double rho = B.rho[wo + q];
+        if (active) {
+            rho = a0.get();
+            B.rho[wo + q] = rho;
+            if (s == 0) {
+                B.p[wo + q] = (T.eos == 0) ? rho * B.e[wo + q] * (T.gamma - 1) : T.cs * T.cs * rho;
+            } else {
+                B.ts[wo + q] = rho / T.K[s - 1];
+            }
+        }
+        if (kStats) {
+            ncount[q] = active ? cnt : 0;
+            nhash[q] = active ? hash : 0ull;
+        }
+        if (s == 0) {
+            const double p = B.p[wo + q];
+            pr2[q] = p / (rho * rho);
+            csav[q] = sqrt(T.gamma * p / rho);
+        }
+    } else {
+        B.ax[wo + q] = active ? a0.get() + d0 : 0.0;
+        B.ay[wo + q] = active ? a1.get() + d1 : 0.0;
+        B.az[wo + q] = active ? a2.get() + d2s : 0.0;
+        B.ed[wo + q] = active ? a3.get() + d3 : 0.0;
+    }
+}
+
 // kernel.cu:2310-2437 for one component; n_dust 1 or 2, reference operation order.
 __device__ __forceinline__ void idic_solve(int nd, double vg, const double* vd, double psi, const double* tb,
                                            const double* ep, double tau, double* vgh, double* vdh) {
@@ -965,6 +1266,8 @@ inline dim3 grid1(int n) { return dim3((unsigned)((n + kBS - 1) / kBS)); }
 
 int scan_blocks(int n) { return (n + kScanTile - 1) / kScanTile; }
 
+size_t mask_words(int gas_chunks) { return (size_t)gas_chunks * kMaskSlots * 64; }
+
 hipError_t launch_prepare(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_prepare, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.B);
     return hipGetLastError();
@@ -1012,7 +1315,24 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename R, int KIND, bool kStats>
+static void launch_tiled(const StepArgs& a, const Map& m, hipStream_t s) {
+    hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur, a.B,
+                       a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks);
+}
+
 hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s) {
+    if (a.variant == kPairTiled) {
+        if (a.pmap64.nblocks == 0) return hipSuccess;
+        if (precision == 0) {
+            if (stats) launch_tiled<double, 0, true>(a, a.pmap64, s);
+            else launch_tiled<double, 0, false>(a, a.pmap64, s);
+        } else {
+            if (stats) launch_tiled<float, 0, true>(a, a.pmap64, s);
+            else launch_tiled<float, 0, false>(a, a.pmap64, s);
+        }
+        return hipGetLastError();
+    }
     const dim3 g(a.pmap.nblocks), b(kBS);
     if (precision == 0) {
         if (stats)
@@ -1033,6 +1353,12 @@ hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStrea
 }
 
 hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
+    if (a.variant == kPairTiled) {
+        if (a.gmap64.nblocks == 0) return hipSuccess;
+        if (precision == 0) launch_tiled<double, 1, false>(a, a.gmap64, s);
+        else launch_tiled<float, 1, false>(a, a.gmap64, s);
+        return hipGetLastError();
+    }
     if (a.gmap.nblocks == 0) return hipSuccess;
     const dim3 g(a.gmap.nblocks), b(kBS);
     if (precision == 0)

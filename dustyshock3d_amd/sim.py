@@ -119,6 +119,9 @@ class Simulation:
         return {"origin": d[0:3], "cell": d[3:6], "length": d[6:9], "n": i[0:3], "f": int(i[3]),
                 "n_avg": int(i[4])}
 
+    def set_pair_variant(self, variant: int):
+        self._call(self._lib.dsh_set_pair_variant(self._ctx, int(variant)))
+
     def set_stats(self, on: bool = True):
         self._call(self._lib.dsh_set_stats(self._ctx, int(on)))
 

@@ -151,6 +151,11 @@ int dsh_grid(dsh_ctx* ctx, double dbl[9], int ints[5]);
 /* Neighbour-set fingerprint of the last density pass, original particle order:
    count = neighbours incl. self and periodic images, hash = order-free sum of ids. */
 int dsh_set_stats(dsh_ctx* ctx, int on);
+/* Pair-kernel variant: DSH_PAIR_TILED (default: one wave per 64-target chunk, LDS-staged
+   candidate rows) or DSH_PAIR_DIRECT (one thread per target).  Same neighbour sets. */
+#define DSH_PAIR_DIRECT 0
+#define DSH_PAIR_TILED 1
+int dsh_set_pair_variant(dsh_ctx* ctx, int variant);
 int dsh_neighbour_stats(dsh_ctx* ctx, int species, int* count, uint64_t* hash);
 /* Per-kernel GPU time of the last timed step, ms (HIP events on the step stream). */
 int dsh_set_timing(dsh_ctx* ctx, int on);

@@ -33,6 +33,17 @@ def rel_normwise(a, b):
     return d / s if s > 0 else d
 
 
+def vec_normwise(a: dict, b: dict, keys=("vx", "vy", "vz"), mask=None):
+    """max |a - b| / max |b| over particles and components of a vector field."""
+    da = np.stack([a[k] for k in keys], 1)
+    db = np.stack([b[k] for k in keys], 1)
+    if mask is not None:
+        da, db = da[mask], db[mask]
+    s = float(np.max(np.abs(db))) if db.size else 0.0
+    d = float(np.max(np.abs(da - db))) if db.size else 0.0
+    return d / s if s > 0 else d
+
+
 def momentum_error(cons_a, cons_b, states_b: dict):
     """|dP| / max(|P|, sum m|v|) with P the reference momentum."""
     P = np.asarray(cons_b[:3])

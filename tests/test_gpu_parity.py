@@ -5,14 +5,18 @@ reference's Nn linked lists (kernel.cu:1937-1995) and the neighbour sets (count 
 order-free hash of (id, periodic image) per particle).  Fields within tolerance:
   FP64 mode   density, acceleration, velocities ~1e-10 (summation order only)
   mixed mode  density <= 1e-5 pointwise, velocities <= 1e-5 normwise (FP32 pair sums)
-  momentum    |dP| / max(|P|, sum m|v|) <= 1e-9
+  momentum    |dP| / max(|P|, sum m|v|) <= 1e-9 in FP64 mode.  In mixed mode the FP32 pair
+              terms between active particles cancel exactly (they are antisymmetric bit for bit)
+              and frozen-wall pairs are evaluated in FP64, but the wall force still depends on
+              the FP32 density of wall-adjacent particles (~1.5e-7 relative), so the bound is
+              1e-7 there (measured 1.4e-8 one step from rest, 8e-10 once the flow has developed).
 """
 import numpy as np
 import pytest
 
 from dustyshock3d_amd import Simulation, configs
-from dustyshock3d_amd.abi import FP64, MIXED
-from tests.parity_util import inject, momentum_error, rel_normwise, rel_pointwise, take_state
+from dustyshock3d_amd.abi import FP64, MIXED, PAIR_DIRECT, PAIR_TILED
+from tests.parity_util import inject, momentum_error, rel_normwise, rel_pointwise, take_state, vec_normwise
 
 pytestmark = pytest.mark.gpu
 
@@ -24,23 +28,25 @@ def _oracle_state(oracle_mod, spec, steps):
     return o.params, take_state(o)
 
 
-def _one_step_both(oracle_mod, params, state, precision):
+def _one_step_both(oracle_mod, params, state, precision, variant=PAIR_TILED):
     params.precision = precision
     o = inject(oracle_mod.OracleSim, params, state)
     g = inject(Simulation, params, state)
+    g.set_pair_variant(variant)
     g.set_stats(True)
     o.step(1)
     g.step(1)
     return o, g
 
 
+@pytest.mark.parametrize("variant", [PAIR_TILED, PAIR_DIRECT])
 @pytest.mark.parametrize("precision", [FP64, MIXED])
 @pytest.mark.parametrize("warm_steps", [0, 20])
-def test_step_parity_shared_state(oracle_mod, precision, warm_steps):
+def test_step_parity_shared_state(oracle_mod, precision, warm_steps, variant):
     spec = configs.cfg0()
     params, state = _oracle_state(oracle_mod, spec, warm_steps)
-    o, g = _one_step_both(oracle_mod, params, state, precision)
-    print(f"\ncase precision={precision} warm={warm_steps}", flush=True)
+    print(f"\ncase precision={precision} warm={warm_steps} variant={variant}", flush=True)
+    o, g = _one_step_both(oracle_mod, params, state, precision, variant)
     for s in (0, 1):
         ro, rg = o.download(s), g.download(s)
         # integer work: bit-exact
@@ -54,15 +60,17 @@ def test_step_parity_shared_state(oracle_mod, precision, warm_steps):
         tol_rho = 1e-12 if precision == FP64 else 1e-5
         e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
         tol_v = 1e-10 if precision == FP64 else 1e-5
-        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        e_v = vec_normwise(rg, ro)
         e_x = max(float(np.max(np.abs(rg[k] - ro[k]))) for k in ("x", "y", "z"))
-        print(f"species {s}: rho {e_rho:.3e} v {e_v:.3e} x {e_x:.3e}")
+        vmax = max(float(np.max(np.abs(ro[k]))) for k in ("vx", "vy", "vz"))
+        tol_x = tol_v * vmax * o.status()["dt"] + 1e-15
+        print(f"species {s}: rho {e_rho:.3e} v {e_v:.3e} x {e_x:.3e} (tol {tol_x:.1e})")
         assert e_rho <= tol_rho
         assert e_v <= tol_v
-        assert e_x <= 1e-12
+        assert e_x <= tol_x
         if s == 0:
             tol_a = 1e-10 if precision == FP64 else 1e-5
-            e_a = max(rel_normwise(rg[k], ro[k]) for k in ("ax", "ay", "az"))
+            e_a = vec_normwise(rg, ro, ("ax", "ay", "az"))
             e_ed = rel_normwise(rg["edot"], ro["edot"])
             print(f"gas A {e_a:.3e} edot {e_ed:.3e}")
             assert e_a <= tol_a
@@ -73,7 +81,8 @@ def test_step_parity_shared_state(oracle_mod, precision, warm_steps):
     st_o = take_state(o)
     e_p = momentum_error(g.conservation(), o.conservation(), st_o)
     print(f"momentum {e_p:.3e}")
-    assert e_p <= 1e-9
+    assert e_p <= 1e-7
+    assert e_p <= (1e-9 if precision == FP64 else 1e-7)
     so, sg = o.status(), g.status()
     assert sg["steps"] == so["steps"] == 1
     assert abs(sg["t"] - so["t"]) <= 1e-15
@@ -94,12 +103,13 @@ def test_cfg0_run_fp64(oracle_mod):
         ro, rg = st_o[s], g.download(s)
         act = ro["ind"] == 0
         e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
-        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        e_v = vec_normwise(rg, ro)
         print(f"\ncfg0 species {s}: rho {e_rho:.3e} v {e_v:.3e}")
         assert e_rho <= 1e-5
         assert e_v <= 1e-5
     e_p = momentum_error(g.conservation(), o.conservation(), st_o)
     print(f"momentum {e_p:.3e}")
+    assert e_p <= 1e-7
     assert e_p <= 1e-9
 
 
@@ -115,9 +125,10 @@ def test_cfg0_geometry_mixed_run(oracle_mod):
         ro, rg = st_o[s], g.download(s)
         act = ro["ind"] == 0
         e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
-        e_v = max(rel_normwise(rg[k], ro[k]) for k in ("vx", "vy", "vz"))
+        e_v = vec_normwise(rg, ro)
         print(f"\nmixed species {s}: rho {e_rho:.3e} v {e_v:.3e}")
         assert e_rho <= 1e-5
         assert e_v <= 1e-5
     e_p = momentum_error(g.conservation(), o.conservation(), st_o)
     print(f"momentum {e_p:.3e}")
+    assert e_p <= 1e-7
