@@ -91,6 +91,22 @@ class TubeSpec(C.Structure):
     ]
 
 
+CLOCK_FN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+
+
+class ServerOpts(C.Structure):
+    """dsh_server_opts."""
+
+    _fields_ = [
+        ("hbm_budget", C.c_double), ("hbm_capacity", C.c_double),
+        ("steps_per_epoch", C.c_int), ("backend", C.c_int), ("pair_variant", C.c_int),
+        ("clock", CLOCK_FN), ("clock_user", C.c_void_p), ("fake_fail_rebuild", C.c_int),
+    ]
+
+
+BACKEND_GPU, BACKEND_FAKE = 0, 1
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED, REQ_FAILED, REQ_REJECTED = range(6)
+
 DOUBLE_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "e", "h",
                  "t_stop", "edot")
 INT_FIELDS = ("ind", "nn", "cell")
@@ -139,6 +155,7 @@ def _declare(lib):
     vp = C.c_void_p
     sig = {
         "dsh_default_params": (C.c_int, [P(Params)]),
+        "dsh_struct_sizes": (C.c_int, [P(C.c_size_t)]),
         "dsh_tube_params": (C.c_int, [P(TubeSpec), P(Params)]),
         "dsh_tube_count": (C.c_int, [P(TubeSpec), _IP]),
         "dsh_tube_ic": (C.c_int, [P(TubeSpec), C.c_int, P(SoaView)]),
@@ -157,6 +174,29 @@ def _declare(lib):
         "dsh_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),
         "dsh_last_error": (C.c_char_p, [vp]),
         "dsh_destroy": (None, [vp]),
+        "dsh_batch_create": (C.c_int, [P(Params), C.c_int, P(vp)]),
+        "dsh_batch_from_specs": (C.c_int, [P(TubeSpec), C.c_int, P(vp)]),
+        "dsh_batch_upload": (C.c_int, [vp, C.c_int, C.c_int, P(SoaView), C.c_int]),
+        "dsh_batch_run": (C.c_int, [vp, C.c_int, _IP]),
+        "dsh_batch_status": (C.c_int, [vp, C.c_int, _DP, _DP, _IP, _IP]),
+        "dsh_batch_download": (C.c_int, [vp, C.c_int, C.c_int, P(SoaView)]),
+        "dsh_batch_conservation": (C.c_int, [vp, C.c_int, _DP]),
+        "dsh_batch_set_pair_variant": (C.c_int, [vp, C.c_int]),
+        "dsh_batch_set_timing": (C.c_int, [vp, C.c_int]),
+        "dsh_batch_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),
+        "dsh_batch_device_bytes": (C.c_int, [vp, _DP]),
+        "dsh_batch_last_error": (C.c_char_p, [vp]),
+        "dsh_batch_destroy": (None, [vp]),
+        "dsh_server_create": (C.c_int, [P(ServerOpts), P(vp)]),
+        "dsh_server_submit": (C.c_int, [vp, P(TubeSpec), C.c_double, P(C.c_uint64)]),
+        "dsh_server_pump": (C.c_int, [vp, _IP, _IP]),
+        "dsh_server_poll": (C.c_int, [vp, C.c_uint64, _IP, _DP, _DP, _DP, _DP]),
+        "dsh_server_result": (C.c_int, [vp, C.c_uint64, C.c_int, P(SoaView)]),
+        "dsh_server_cancel": (C.c_int, [vp, C.c_uint64]),
+        "dsh_server_footprint": (C.c_int, [P(TubeSpec), _DP]),
+        "dsh_server_stats": (C.c_int, [vp, _DP, _IP, _IP]),
+        "dsh_server_last_error": (C.c_char_p, [vp]),
+        "dsh_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

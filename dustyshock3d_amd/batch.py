@@ -1,0 +1,162 @@
+"""Batches of independent tubes and the continuous-batching server, over the C ABI."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .abi import DshError, ServerOpts, TubeSpec, empty_arrays, make_view
+from .sim import tube_count
+
+
+def _times(fn, handle) -> dict:
+    nmax, ln = 32, 32
+    ms = np.zeros(nmax)
+    names = C.create_string_buffer(nmax * ln)
+    n = fn(handle, ms.ctypes.data_as(C.POINTER(C.c_double)), names, nmax, ln)
+    raw = names.raw
+    return {raw[i * ln:(i + 1) * ln].split(b"\0")[0].decode(): float(ms[i]) for i in range(min(max(n, 0), nmax))}
+
+
+class Batch:
+    """Many tubes stepped together: one launch sequence per step for the whole batch."""
+
+    def __init__(self, specs: Sequence[TubeSpec]):
+        self._lib = abi.lib()
+        self.specs = list(specs)
+        arr = (TubeSpec * len(self.specs))(*self.specs)
+        self._h = C.c_void_p()
+        rc = self._lib.dsh_batch_from_specs(arr, len(self.specs), C.byref(self._h))
+        if rc != abi.OK:
+            raise DshError(rc, "dsh_batch_from_specs")
+        self.counts = [tube_count(s) for s in self.specs]
+
+    def _call(self, rc):
+        if rc != abi.OK:
+            m = self._lib.dsh_batch_last_error(self._h)
+            raise DshError(rc, m.decode() if m else "")
+
+    def run(self, nsteps: int) -> int:
+        nd = C.c_int(0)
+        self._call(self._lib.dsh_batch_run(self._h, nsteps, C.byref(nd)))
+        return nd.value
+
+    def status(self, tube: int) -> dict:
+        t, dt = C.c_double(), C.c_double()
+        steps, done = C.c_int(), C.c_int()
+        self._call(self._lib.dsh_batch_status(self._h, tube, C.byref(t), C.byref(dt), C.byref(steps), C.byref(done)))
+        return {"t": t.value, "dt": dt.value, "steps": steps.value, "done": bool(done.value)}
+
+    def download(self, tube: int, species: int, fields=abi.ALL_FIELDS) -> dict:
+        arrs = empty_arrays(self.counts[tube], fields)
+        v = make_view(arrs)
+        self._call(self._lib.dsh_batch_download(self._h, tube, species, C.byref(v)))
+        return arrs
+
+    def conservation(self, tube: int) -> np.ndarray:
+        out = np.zeros(4)
+        self._call(self._lib.dsh_batch_conservation(self._h, tube, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def set_pair_variant(self, v: int):
+        self._call(self._lib.dsh_batch_set_pair_variant(self._h, int(v)))
+
+    def set_timing(self, on: bool = True):
+        self._call(self._lib.dsh_batch_set_timing(self._h, int(on)))
+
+    def kernel_times(self) -> dict:
+        return _times(self._lib.dsh_batch_kernel_times, self._h)
+
+    def device_bytes(self) -> float:
+        b = C.c_double()
+        self._call(self._lib.dsh_batch_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def close(self):
+        if self._h:
+            self._lib.dsh_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(spec: TubeSpec) -> float:
+    b = C.c_double()
+    rc = abi.lib().dsh_server_footprint(C.byref(spec), C.byref(b))
+    if rc != abi.OK:
+        raise DshError(rc, "dsh_server_footprint")
+    return b.value
+
+
+class Server:
+    """Continuous batching of tube requests (GPU engine or the host-only fake device)."""
+
+    def __init__(self, hbm_budget: float, steps_per_epoch: int = 10, backend: int = abi.BACKEND_GPU,
+                 clock: Optional[Callable[[], float]] = None, hbm_capacity: float = 0.0,
+                 pair_variant: int = abi.PAIR_TILED, fake_fail_rebuild: int = 0):
+        self._lib = abi.lib()
+        self._clock_cb = abi.CLOCK_FN(lambda _u: float(clock())) if clock else abi.CLOCK_FN()
+        o = ServerOpts(hbm_budget=hbm_budget, hbm_capacity=hbm_capacity, steps_per_epoch=steps_per_epoch,
+                       backend=backend, pair_variant=pair_variant, clock=self._clock_cb, clock_user=None,
+                       fake_fail_rebuild=fake_fail_rebuild)
+        self._h = C.c_void_p()
+        rc = self._lib.dsh_server_create(C.byref(o), C.byref(self._h))
+        if rc != abi.OK:
+            raise DshError(rc, "dsh_server_create")
+        self.specs = {}
+
+    def _call(self, rc):
+        if rc != abi.OK:
+            m = self._lib.dsh_server_last_error(self._h)
+            raise DshError(rc, m.decode() if m else "")
+
+    def submit(self, spec: TubeSpec, arrival: float) -> int:
+        i = C.c_uint64()
+        self._call(self._lib.dsh_server_submit(self._h, C.byref(spec), float(arrival), C.byref(i)))
+        self.specs[i.value] = spec
+        return i.value
+
+    def pump(self):
+        r, q = C.c_int(), C.c_int()
+        self._call(self._lib.dsh_server_pump(self._h, C.byref(r), C.byref(q)))
+        return r.value, q.value
+
+    def poll(self, rid: int) -> dict:
+        st = C.c_int()
+        a, ad, f, ps = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self._call(self._lib.dsh_server_poll(self._h, rid, C.byref(st), C.byref(a), C.byref(ad), C.byref(f),
+                                             C.byref(ps)))
+        return {"state": st.value, "arrival": a.value, "admitted": ad.value, "finished": f.value,
+                "particle_steps": ps.value}
+
+    def result(self, rid: int, species: int, fields=abi.DOUBLE_FIELDS + ("ind",)) -> dict:
+        arrs = empty_arrays(tube_count(self.specs[rid]), fields)
+        v = make_view(arrs)
+        self._call(self._lib.dsh_server_result(self._h, rid, species, C.byref(v)))
+        return arrs
+
+    def cancel(self, rid: int):
+        self._call(self._lib.dsh_server_cancel(self._h, rid))
+
+    def stats(self) -> dict:
+        b = C.c_double()
+        r, e = C.c_int(), C.c_int()
+        self._call(self._lib.dsh_server_stats(self._h, C.byref(b), C.byref(r), C.byref(e)))
+        return {"bytes_in_use": b.value, "rebuilds": r.value, "epochs": e.value}
+
+    def close(self):
+        if self._h:
+            self._lib.dsh_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -46,7 +46,37 @@ void build_grid(const dsh_params& P, double hmin, TubeConst& T) {
 }
 
 Engine::Engine(const std::vector<dsh_params>& params)
-    : params_(params), staged_(params.size() * kMaxSpecies) {}
+    : params_(params), staged_(params.size() * kMaxSpecies), migr_(params.size()) {}
+
+int Engine::adopt(int tube, const Engine& src, int src_tube) {
+    if (prepared_) return fail(DSH_ERR_STATE, "adopt after the first step");
+    if (tube < 0 || tube >= ntubes()) return fail(DSH_ERR_ARG, "tube index out of range");
+    if (!src.prepared() || src_tube < 0 || src_tube >= src.ntubes())
+        return fail(DSH_ERR_ARG, "source tube is not on a prepared engine");
+    if (src.params_[src_tube].precision != params_[tube].precision ||
+        src.params_[src_tube].n_dust != params_[tube].n_dust)
+        return fail(DSH_ERR_ARG, "adopted tube must keep its parameters");
+    const TubeConst& C = src.tc_h_[src_tube];
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        HostSpecies& q = staged_[(size_t)tube * kMaxSpecies + s];
+        q = HostSpecies{};
+        if (s < C.nsp) {
+            q.n = C.np[s];
+            q.set = true;
+        }
+    }
+    migr_[tube] = Migrant{&src, src_tube};
+    return DSH_OK;
+}
+
+int Engine::device_fields(int tube, int species, int* cur_out, long long* off_out) const {
+    if (!prepared_ || tube < 0 || tube >= ntubes() || species < 0 || species >= tc_h_[tube].nsp) return DSH_ERR_ARG;
+    int cur = 0;
+    if (hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return DSH_ERR_HIP;
+    *cur_out = cur;
+    *off_out = tc_h_[tube].p_off[species];
+    return DSH_OK;
+}
 
 Engine::~Engine() {
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
@@ -141,12 +171,16 @@ int Engine::prepare() {
     for (int t = 0; t < T; ++t) {
         const dsh_params& P = params_[t];
         TubeConst& C = tc_h_[t];
-        double hmin = 1e300;
-        for (int s = 0; s <= P.n_dust; ++s) {
-            const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
-            for (int i = 0; i < q.n; ++i) hmin = std::min(hmin, q.h[i]);
+        if (migr_[t].src) {
+            C = migr_[t].src->tc_h_[migr_[t].tube];   // keeps the grid the tube was started on
+        } else {
+            double hmin = 1e300;
+            for (int s = 0; s <= P.n_dust; ++s) {
+                const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
+                for (int i = 0; i < q.n; ++i) hmin = std::min(hmin, q.h[i]);
+            }
+            build_grid(P, hmin, C);
         }
-        build_grid(P, hmin, C);
         C.nsp = 1 + P.n_dust;
         C.gamma = P.gamma;
         C.cs = P.cs;
@@ -277,9 +311,39 @@ int Engine::prepare() {
     const int zero = 0;
     if ((rc = hip_check(hipMemcpy(a_.cur, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D cur"))) return rc;
 
-    // upload into buffer 0, original order, id = index
+    // adopted tubes: device-to-device copy of the source's current buffer (its sorted
+    // order and ids are kept; the first step re-sorts to the same order)
+    std::vector<TubeDyn> adopted_dyn;
+    for (int t = 0; t < T; ++t) {
+        if (!migr_[t].src) continue;
+        const Engine& S = *migr_[t].src;
+        const int st = migr_[t].tube;
+        if ((rc = hip_check(hipStreamSynchronize(S.stream_), "source sync"))) return rc;
+        int scur = 0;
+        if ((rc = hip_check(hipMemcpy(&scur, S.a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
+        for (int s = 0; s < tc_h_[t].nsp; ++s) {
+            const size_t n = (size_t)tc_h_[t].np[s];
+            const size_t so = (size_t)scur * S.ntot_ + (size_t)S.tc_h_[st].p_off[s];
+            const size_t off = (size_t)tc_h_[t].p_off[s];
+            const PBuf& SB = S.a_.B;
+            const double* sd[] = {SB.x, SB.y, SB.z, SB.vx, SB.vy, SB.vz, SB.m, SB.h, SB.e, SB.rho, SB.p, SB.ts};
+            double* dd[] = {B.x, B.y, B.z, B.vx, B.vy, B.vz, B.m, B.h, B.e, B.rho, B.p, B.ts};
+            for (int f = 0; f < 12; ++f)
+                if ((rc = hip_check(hipMemcpy(dd[f] + off, sd[f] + so, n * sizeof(double), hipMemcpyDeviceToDevice),
+                                    "D2D field")))
+                    return rc;
+            if ((rc = hip_check(hipMemcpy(B.ind + off, SB.ind + so, n * sizeof(int), hipMemcpyDeviceToDevice),
+                                "D2D ind")))
+                return rc;
+            if ((rc = hip_check(hipMemcpy(B.id + off, SB.id + so, n * sizeof(int), hipMemcpyDeviceToDevice),
+                                "D2D id")))
+                return rc;
+        }
+    }
+    // uploaded tubes: buffer 0, original order, id = index
     for (int t = 0; t < T; ++t)
         for (int s = 0; s < tc_h_[t].nsp; ++s) {
+            if (migr_[t].src) continue;
             const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
             const size_t off = (size_t)tc_h_[t].p_off[s];
             const size_t n = (size_t)q.n;
@@ -311,6 +375,26 @@ int Engine::prepare() {
         }
     if ((rc = hip_check(launch_prepare(a_, stream_), "k_prepare"))) return rc;
     if ((rc = hip_check(hipStreamSynchronize(stream_), "prepare sync"))) return rc;
+    // adopted tubes continue with their own clock and step size
+    for (int t = 0; t < T; ++t) {
+        if (!migr_[t].src) continue;
+        TubeDyn sd{};
+        if ((rc = hip_check(hipMemcpy(&sd, migr_[t].src->a_.td + migr_[t].tube, sizeof(TubeDyn),
+                                      hipMemcpyDeviceToHost),
+                            "D2H source dyn")))
+            return rc;
+        TubeDyn d{};
+        d.t = sd.t;
+        d.dt = sd.dt;
+        d.steps = sd.steps;
+        d.done = sd.done;
+        d.final_step = sd.final_step;
+        d.err = sd.err;
+        d.done_cur = 0;
+        for (int c = 0; c < 4; ++c) d.cons[c] = sd.cons[c];
+        if ((rc = hip_check(hipMemcpy(a_.td + t, &d, sizeof(TubeDyn), hipMemcpyHostToDevice), "H2D dyn"))) return rc;
+        migr_[t].src = nullptr;
+    }
     prepared_ = true;
     return refresh_dyn();
 }

@@ -30,6 +30,12 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     int upload(int tube, int species, const dsh_soa_view* v, int pm);
+    // Take tube src_tube of a prepared engine, state and clock, without a host round
+    // trip: at prepare() its arrays are copied device to device and its grid is kept.
+    int adopt(int tube, const Engine& src, int src_tube);
+    // Host view of a tube's device arrays in the current buffer (for device-side copies).
+    int device_fields(int tube, int species, int* cur_out, long long* off_out) const;
+    const PBuf& buffers() const { return a_.B; }
     int prepare();
     // Launch nsteps steps (no host synchronisation in between) and wait for them.
     int run(int nsteps);
@@ -71,6 +77,11 @@ private:
 
     std::vector<dsh_params> params_;
     std::vector<HostSpecies> staged_;   // [tube * kMaxSpecies + species]
+    struct Migrant {
+        const Engine* src = nullptr;
+        int tube = -1;
+    };
+    std::vector<Migrant> migr_;
     std::vector<TubeConst> tc_h_;
     std::vector<TubeDyn> dyn_h_;
     std::vector<Seg> pseg_, gseg_, cseg_, aseg_;

@@ -72,6 +72,15 @@ bool spec_ok(const dsh_tube_spec* t) {
 
 extern "C" {
 
+int dsh_struct_sizes(size_t out[4]) {
+    if (!out) return DSH_ERR_ARG;
+    out[0] = sizeof(dsh_params);
+    out[1] = sizeof(dsh_soa_view);
+    out[2] = sizeof(dsh_tube_spec);
+    out[3] = sizeof(dsh_server_opts);
+    return DSH_OK;
+}
+
 int dsh_default_params(dsh_params* p) {
     if (!p) return DSH_ERR_ARG;
     std::memset(p, 0, sizeof(*p));

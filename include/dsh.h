@@ -130,6 +130,8 @@ int dsh_tube_count(const dsh_tube_spec* spec, int* n_per_species);
 int dsh_tube_ic(const dsh_tube_spec* spec, int species, const dsh_soa_view* out);
 /* Init.txt reader (kernel.cu:1705-1742); extension lines after line 18 are optional. */
 int dsh_read_init(const char* path, dsh_params* out);
+/* sizeof of dsh_params, dsh_soa_view, dsh_tube_spec, dsh_server_opts (binding checks) */
+int dsh_struct_sizes(size_t out[4]);
 
 /* ---- single-tube context (replaces kernel.cu:1793-1926 set-up and 1929-2566 loop) ---- */
 typedef struct dsh_ctx dsh_ctx;
@@ -162,6 +164,65 @@ int dsh_set_timing(dsh_ctx* ctx, int on);
 int dsh_kernel_times(dsh_ctx* ctx, double* ms, char* names, int max_kernels, int name_len);
 const char* dsh_last_error(dsh_ctx* ctx);
 void dsh_destroy(dsh_ctx* ctx);
+
+/* ---- batch of independent tubes (no reference counterpart: the reference runs one
+   problem per process; the batch runs many in one launch sequence per step) ---- */
+typedef struct dsh_batch dsh_batch;
+/* ntubes tubes with their own parameters; every tube must be uploaded before stepping */
+int dsh_batch_create(const dsh_params* params, int ntubes, dsh_batch** out);
+/* convenience: parameters and seeded set-up of every tube from its spec (host set-up + upload) */
+int dsh_batch_from_specs(const dsh_tube_spec* specs, int ntubes, dsh_batch** out);
+int dsh_batch_upload(dsh_batch* b, int tube, int species, const dsh_soa_view* view, int pm);
+/* advance every unfinished tube by up to nsteps steps; *ndone = finished tubes */
+int dsh_batch_run(dsh_batch* b, int nsteps, int* ndone);
+int dsh_batch_status(dsh_batch* b, int tube, double* t, double* dt, int* steps, int* done);
+int dsh_batch_download(dsh_batch* b, int tube, int species, const dsh_soa_view* view);
+int dsh_batch_conservation(dsh_batch* b, int tube, double out[4]);
+int dsh_batch_set_pair_variant(dsh_batch* b, int variant);
+int dsh_batch_set_timing(dsh_batch* b, int on);
+int dsh_batch_kernel_times(dsh_batch* b, double* ms, char* names, int max_kernels, int name_len);
+/* bytes of device memory the batch holds */
+int dsh_batch_device_bytes(dsh_batch* b, double* bytes);
+const char* dsh_batch_last_error(dsh_batch* b);
+void dsh_batch_destroy(dsh_batch* b);
+
+/* ---- serving: continuous batching of tube requests (no reference counterpart) ----
+   Requests queue in arrival order.  Each pump() admits queued requests while their HBM
+   footprint fits the budget (and the rebuild transient fits the capacity), rebuilds the running batch (running tubes move device to
+   device, keeping their state and clock), advances it steps_per_epoch steps and retires
+   finished tubes, copying their results to host memory. */
+typedef struct dsh_server dsh_server;
+typedef double (*dsh_clock_fn)(void* user);
+#define DSH_BACKEND_GPU 0
+#define DSH_BACKEND_FAKE 1     /* host-only simulated device, for scheduler tests */
+#define DSH_REQ_QUEUED 0
+#define DSH_REQ_RUNNING 1
+#define DSH_REQ_DONE 2
+#define DSH_REQ_CANCELLED 3
+#define DSH_REQ_FAILED 4
+#define DSH_REQ_REJECTED 5     /* larger than the budget */
+typedef struct dsh_server_opts {
+    double hbm_budget;         /* bytes the running batch may hold */
+    double hbm_capacity;       /* bytes the old and the new batch may hold together during a
+                                  rebuild; 0: 90% of device memory (GPU) or 2 x budget (fake) */
+    int steps_per_epoch;
+    int backend;               /* DSH_BACKEND_GPU or DSH_BACKEND_FAKE */
+    int pair_variant;
+    dsh_clock_fn clock;        /* NULL: monotonic wall clock in seconds */
+    void* clock_user;
+    int fake_fail_rebuild;     /* fake backend: the n-th rebuild (1-based) fails; 0 never */
+} dsh_server_opts;
+int dsh_server_create(const dsh_server_opts* opts, dsh_server** out);
+int dsh_server_submit(dsh_server* sv, const dsh_tube_spec* spec, double arrival, uint64_t* id);
+int dsh_server_pump(dsh_server* sv, int* running, int* queued);
+int dsh_server_poll(dsh_server* sv, uint64_t id, int* state, double* arrival, double* admitted,
+                    double* finished, double* particle_steps);
+int dsh_server_result(dsh_server* sv, uint64_t id, int species, const dsh_soa_view* view);
+int dsh_server_cancel(dsh_server* sv, uint64_t id);
+int dsh_server_footprint(const dsh_tube_spec* spec, double* bytes);
+int dsh_server_stats(dsh_server* sv, double* bytes_in_use, int* rebuilds, int* epochs);
+const char* dsh_server_last_error(dsh_server* sv);
+void dsh_server_destroy(dsh_server* sv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,106 @@
+"""Scheduling logic of the continuous-batching server on the host-only fake device:
+arrival-order admission under the HBM budget, rejection, cancellation, recovery from a
+failed rebuild, and the time accounting the p50/p99 report uses (injected clock)."""
+import pytest
+
+from dustyshock3d_amd import abi, configs
+from dustyshock3d_amd.batch import Server, footprint
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1.0   # every clock read advances one tick
+        return self.t
+
+
+def small(i, steps=30):
+    return configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=steps, tube_index=i)
+
+
+def run_until_idle(sv, limit=1000):
+    for _ in range(limit):
+        r, q = sv.pump()
+        if r == 0 and q == 0:
+            return
+    raise AssertionError("server did not drain")
+
+
+def test_admission_respects_budget_and_order():
+    fp = footprint(small(0))
+    sv = Server(hbm_budget=2.5 * fp, hbm_capacity=5 * fp, steps_per_epoch=10, backend=abi.BACKEND_FAKE,
+                clock=Clock())
+    ids = [sv.submit(small(i), arrival=0.0) for i in range(5)]
+    r, q = sv.pump()
+    assert (r, q) == (2, 3)                      # two fit the budget
+    assert sv.stats()["bytes_in_use"] <= 2.5 * fp
+    run_until_idle(sv)
+    polls = [sv.poll(i) for i in ids]
+    assert all(p["state"] == abi.REQ_DONE for p in polls)
+    adm = [p["admitted"] for p in polls]
+    assert adm == sorted(adm)                     # admitted in arrival order
+    assert all(p["finished"] > p["admitted"] >= p["arrival"] for p in polls)
+    assert all(p["particle_steps"] > 0 for p in polls)
+
+
+def test_oversized_request_is_rejected_not_blocking():
+    fp = footprint(small(0))
+    big = configs.cfg1()
+    sv = Server(hbm_budget=3 * fp, steps_per_epoch=10, backend=abi.BACKEND_FAKE, clock=Clock())
+    a = sv.submit(big, 0.0)
+    b = sv.submit(small(1), 0.0)
+    sv.pump()
+    assert sv.poll(a)["state"] == abi.REQ_REJECTED
+    assert sv.poll(b)["state"] in (abi.REQ_RUNNING, abi.REQ_DONE)
+    run_until_idle(sv)
+    assert sv.poll(b)["state"] == abi.REQ_DONE
+
+
+def test_cancel_queued_and_running():
+    fp = footprint(small(0))
+    sv = Server(hbm_budget=1.5 * fp, hbm_capacity=4 * fp, steps_per_epoch=5, backend=abi.BACKEND_FAKE,
+                clock=Clock())
+    a = sv.submit(small(0, 100), 0.0)
+    b = sv.submit(small(1, 100), 0.0)
+    sv.pump()
+    assert sv.poll(a)["state"] == abi.REQ_RUNNING and sv.poll(b)["state"] == abi.REQ_QUEUED
+    sv.cancel(b)
+    sv.cancel(a)
+    run_until_idle(sv)
+    assert sv.poll(a)["state"] == abi.REQ_CANCELLED
+    assert sv.poll(b)["state"] == abi.REQ_CANCELLED
+    assert sv.stats()["bytes_in_use"] == 0
+
+
+def test_failed_rebuild_keeps_running_batch_and_retries():
+    fp = footprint(small(0))
+    sv = Server(hbm_budget=4 * fp, hbm_capacity=8 * fp, steps_per_epoch=10, backend=abi.BACKEND_FAKE,
+                clock=Clock(), fake_fail_rebuild=2)
+    a = sv.submit(small(0, 50), 0.0)
+    sv.pump()                                  # rebuild 1: a admitted
+    b = sv.submit(small(1, 20), 5.0)
+    sv.pump()                                  # rebuild 2 fails: a keeps running, b stays queued
+    assert sv.poll(a)["state"] == abi.REQ_RUNNING
+    assert sv.poll(b)["state"] == abi.REQ_QUEUED
+    run_until_idle(sv)                         # rebuild 3 admits b
+    assert sv.poll(a)["state"] == abi.REQ_DONE and sv.poll(b)["state"] == abi.REQ_DONE
+
+
+def test_new_arrivals_join_a_running_batch():
+    """Continuous batching: a request arriving while another runs starts before it ends."""
+    fp = footprint(small(0))
+    sv = Server(hbm_budget=4 * fp, hbm_capacity=8 * fp, steps_per_epoch=10, backend=abi.BACKEND_FAKE,
+                clock=Clock())
+    a = sv.submit(small(0, 100), 0.0)
+    sv.pump()
+    sv.pump()
+    b = sv.submit(small(1, 20), 10.0)
+    sv.pump()
+    assert sv.poll(b)["state"] in (abi.REQ_RUNNING, abi.REQ_DONE)
+    assert sv.poll(a)["state"] == abi.REQ_RUNNING
+    run_until_idle(sv)
+    pa, pb = sv.poll(a), sv.poll(b)
+    assert pb["finished"] < pa["finished"]
+    assert pytest.approx(pb["particle_steps"]) == 20 * 2 * 900
