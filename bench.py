@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -155,5 +157,160 @@ def main():
     print(json.dumps(out))
 
 
+def run_alone(spec, variant=1):
+    """One tube alone on the GPU (the plain one-tube-at-a-time form): particle-steps, seconds."""
+    from dustyshock3d_amd import Simulation
+
+    s = Simulation.from_spec(spec)
+    s.set_pair_variant(variant)
+    s.step(0)
+    t0 = time.perf_counter()
+    while not s.status()["done"]:
+        s.step(8)
+    dt = time.perf_counter() - t0
+    st = s.status()
+    return sum(s.counts.values()) * st["steps"], dt
+
+
+def sweep_mode(args):
+    """cfg3: 512 independent tubes in one batch, each run to t=0.2, against the same tubes
+    run one at a time (a sample of every `serial_stride`-th tube) on the same GPU."""
+    from dustyshock3d_amd import configs
+    from dustyshock3d_amd.batch import Batch
+
+    specs = configs.cfg3(args.tubes)
+    t_setup = time.perf_counter()
+    b = Batch(specs)
+    b.set_pair_variant(args.variant)
+    b.run(0)
+    setup = time.perf_counter() - t_setup
+    t0 = time.perf_counter()
+    finish = {}
+    while len(finish) < len(specs):
+        b.run(args.chunk)
+        now = time.perf_counter() - t0
+        for i in range(len(specs)):
+            if i not in finish and b.status(i)["done"]:
+                finish[i] = now
+    wall = time.perf_counter() - t0
+    ps = sum(2 * b.counts[i] * b.status(i)["steps"] for i in range(len(specs)))
+    steps = [b.status(i)["steps"] for i in range(len(specs))]
+    del b
+    sample = list(range(0, len(specs), args.serial_stride))
+    sps, sdt = 0.0, 0.0
+    for i in sample:
+        p, d = run_alone(specs[i], args.variant)
+        sps += p
+        sdt += d
+    serial_rate = sps / sdt
+    tts = np.array(sorted(finish.values()))
+    out = {
+        "metric": "sweep particle-steps/s (cfg3, 512 tubes in one batch)",
+        "value": ps / wall, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_serial_gpu": (ps / wall) / serial_rate,
+        "serial_gpu_particle_steps_per_s": serial_rate,
+        "serial_sample": f"{len(sample)} of {len(specs)} tubes run one at a time, {sdt:.1f} s",
+        "wall_s": wall, "setup_s": setup, "particle_steps": ps,
+        "tubes": len(specs), "steps_min_max": [min(steps), max(steps)],
+        "time_to_solution_s": {"p50": float(np.percentile(tts, 50)), "p99": float(np.percentile(tts, 99))},
+        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
+        "config": {"workload": "cfg3 sweep: y,z in [0,0.05], dx log-uniform [0.0025,0.01], K log-uniform "
+                               "[0.01,1000], dust-to-gas uniform [0.01,10], each to t=0.2",
+                   "pair_kernels": "tiled" if args.variant == 1 else "direct"},
+    }
+    print(json.dumps(out))
+
+
+def serve_mode(args):
+    """cfg4: Poisson arrivals at a fraction of the measured saturated throughput, 95% sweep tubes
+    and 5% large tubes, continuous batching under an HBM budget; the same request stream replayed
+    one request at a time (FIFO, measured service times) gives the serial comparison."""
+    from dustyshock3d_amd import abi, configs
+    from dustyshock3d_amd.batch import Server
+
+    rng = np.random.default_rng(args.seed)
+    reqs = []
+    for i in range(args.requests):
+        if rng.uniform() < 0.05:
+            reqs.append(configs.large_tube(i, precision=abi.MIXED))
+        else:
+            reqs.append(configs.sweep_tube(10000 + i, precision=abi.MIXED))
+    # service time of every request alone (serial plain form), measured
+    service, psteps = [], []
+    for sp in reqs:
+        p, d = run_alone(sp, args.variant)
+        service.append(d)
+        psteps.append(p)
+    service = np.array(service)
+    psteps = np.array(psteps)
+    sat_rate = args.sat_rate if args.sat_rate > 0 else psteps.sum() / service.sum()
+    # arrivals at `load` of the saturated rate, in requests/s
+    lam = args.load * sat_rate / psteps.mean()
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(reqs)))
+    # serial FIFO replay with the measured service times
+    t_free, serial_tts = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, t_free)
+        t_free = start + s
+        serial_tts.append(t_free - a)
+    serial_tts = np.array(serial_tts)
+    # continuous batching
+    t0 = time.perf_counter()
+    clock = lambda: time.perf_counter() - t0  # noqa: E731
+    sv = Server(hbm_budget=args.budget, steps_per_epoch=args.epoch, pair_variant=args.variant, clock=clock)
+    ids, k = [], 0
+    while True:
+        now = clock()
+        while k < len(reqs) and arrivals[k] <= now:
+            ids.append(sv.submit(reqs[k], float(arrivals[k])))
+            k += 1
+        r, q = sv.pump()
+        if k == len(reqs) and r == 0 and q == 0:
+            break
+        if r == 0 and q == 0 and k < len(reqs):
+            time.sleep(max(0.0, min(0.01, arrivals[k] - clock())))
+    wall = clock()
+    polls = [sv.poll(i) for i in ids]
+    done = [p for p in polls if p["state"] == abi.REQ_DONE]
+    tts = np.array([p["finished"] - p["arrival"] for p in done])
+    ps = sum(p["particle_steps"] for p in done)
+    st = sv.stats()
+    out = {
+        "metric": "serving particle-steps/s and time-to-solution (cfg4 stream, continuous batching)",
+        "value": ps / wall, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "requests": len(reqs), "completed": len(done),
+        "large_requests": int(sum(1 for sp in reqs if sp.y_width > 0.05)),
+        "offered_load": args.load, "arrival_rate_per_s": lam,
+        "saturated_serial_rate": sat_rate,
+        "time_to_solution_s": {"p50": float(np.percentile(tts, 50)), "p99": float(np.percentile(tts, 99))},
+        "serial_fifo_time_to_solution_s": {"p50": float(np.percentile(serial_tts, 50)),
+                                           "p99": float(np.percentile(serial_tts, 99))},
+        "serial_particle_steps_per_s": float(psteps.sum() / service.sum()),
+        "wall_s": wall, "rebuilds": st["rebuilds"], "epochs": st["epochs"], "hbm_budget": args.budget,
+        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
+        "config": {"workload": "cfg4: Poisson arrivals, 95% cfg3-distribution tubes, 5% large tubes "
+                               "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch},
+    }
+    print(json.dumps(out))
+
+
 if __name__ == "__main__":
-    main()
+    if "--mode" in sys.argv:
+        ap = argparse.ArgumentParser()
+        ap.add_argument("--mode", choices=["sweep", "serve"], required=True)
+        ap.add_argument("--gpus", type=int, default=1)
+        ap.add_argument("--variant", type=int, default=1)
+        ap.add_argument("--tubes", type=int, default=512)
+        ap.add_argument("--chunk", type=int, default=16)
+        ap.add_argument("--serial-stride", type=int, default=8)
+        ap.add_argument("--requests", type=int, default=60)
+        ap.add_argument("--load", type=float, default=0.8)
+        ap.add_argument("--sat-rate", type=float, default=0.0)
+        ap.add_argument("--budget", type=float, default=200e9)
+        ap.add_argument("--epoch", type=int, default=8)
+        ap.add_argument("--seed", type=int, default=7)
+        a = ap.parse_args()
+        import numpy as np  # noqa: F811
+        (sweep_mode if a.mode == "sweep" else serve_mode)(a)
+    else:
+        main()

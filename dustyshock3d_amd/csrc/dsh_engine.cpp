@@ -218,8 +218,10 @@ int Engine::prepare() {
                 C.p_off[s] = 0;
                 C.np[s] = 0;
                 C.cell_off[s] = 0;
+                C.pseg[s] = -1;
                 continue;
             }
+            C.pseg[s] = (int)pseg_.size();
             const int n = staged_[(size_t)t * kMaxSpecies + s].n;
             C.p_off[s] = (int)poff;
             C.np[s] = n;
@@ -373,7 +375,9 @@ int Engine::prepare() {
                                         "H2D zero")))
                         return rc;
         }
-    if ((rc = hip_check(launch_prepare(a_, stream_), "k_prepare"))) return rc;
+    double* prep = nullptr;
+    if ((rc = alloc(&prep, 5 * (size_t)a_.pmap.nblocks))) return rc;
+    if ((rc = hip_check(launch_prepare(a_, prep, stream_), "k_prepare"))) return rc;
     if ((rc = hip_check(hipStreamSynchronize(stream_), "prepare sync"))) return rc;
     // adopted tubes continue with their own clock and step size
     for (int t = 0; t < T; ++t) {
@@ -545,7 +549,8 @@ int Engine::download(int tube, int species, const dsh_soa_view* v) {
     if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
     const TubeConst& C = tc_h_[tube];
     const TubeDyn& D = dyn_h_[tube];
-    const int derived = D.done ? D.done_cur : cur;
+    const int derived = cur;   // finished tubes carry their last A along when re-sorted
+    (void)D;
     const size_t n = (size_t)C.np[species];
     const size_t off = (size_t)C.p_off[species];
     std::vector<int> ids(n), idd(n);
@@ -610,7 +615,8 @@ int Engine::neighbour_stats(int tube, int species, int* count, uint64_t* hash) {
     int cur = 0;
     if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
     const TubeDyn& D = dyn_h_[tube];
-    const int derived = D.done ? D.done_cur : cur;
+    const int derived = cur;   // finished tubes carry their last A along when re-sorted
+    (void)D;
     const size_t n = (size_t)tc_h_[tube].np[species];
     const size_t off = (size_t)tc_h_[tube].p_off[species];
     std::vector<int> ids(n), c(n);

@@ -47,7 +47,8 @@ struct StepArgs {
     int variant;
 };
 
-hipError_t launch_prepare(const StepArgs& a, hipStream_t s);
+// scratch: 5 doubles per particle block of pmap
+hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s);
 hipError_t launch_begin(const StepArgs& a, hipStream_t s);
 hipError_t launch_key(const StepArgs& a, hipStream_t s);
 hipError_t launch_scan(const StepArgs& a, hipStream_t s);

@@ -198,30 +198,24 @@ __device__ __forceinline__ ItemPos locate(const Map& mp, int per_block) {
 
 // ---------------------------------------------------------------- kernels
 
-// Initial time step and conservation sums from the uploaded state (buffer 0).
-__global__ __launch_bounds__(kBS) void k_prepare(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                 PBuf B) {
+// Initial time step and conservation sums from the uploaded state (buffer 0): per-block
+// partials over every particle, then one block per tube reduces its blocks in order.
+__global__ __launch_bounds__(kBS) void k_prepare_part(Map mp, const TubeConst* __restrict__ tc, PBuf B,
+                                                      double* __restrict__ part) {
     __shared__ double sh[5][kBS / 64];
-    const int t = blockIdx.x;
-    const TubeConst& T = tc[t];
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    const TubeConst& T = tc[ip.sg.tube];
     double mn = 1e300, px = 0, py = 0, pz = 0, E = 0;
-    for (int s = 0; s < T.nsp; ++s) {
-        const int off = T.p_off[s], n = T.np[s];
-        for (int i = threadIdx.x; i < n; i += kBS) {
-            const int p = off + i;
-            const double vx = B.vx[p], vy = B.vy[p], vz = B.vz[p], m = B.m[p];
-            px += m * vx;
-            py += m * vy;
-            pz += m * vz;
-            double k = 0.5 * m * (vx * vx + vy * vy + vz * vz);
-            if (s == 0) k += m * B.e[p];
-            E += k;
-            if (B.ind[p] == 0) {
-                const double v = sqrt(vx * vx + vy * vy + vz * vz);
-                const double r = B.h[p] / (T.cs + v);
-                mn = fmin(mn, r);
-            }
-        }
+    if (li < ip.sg.n) {
+        const int p = ip.sg.off + li;
+        const double vx = B.vx[p], vy = B.vy[p], vz = B.vz[p], m = B.m[p];
+        px = m * vx;
+        py = m * vy;
+        pz = m * vz;
+        E = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+        if (ip.sg.species == 0) E += m * B.e[p];
+        if (B.ind[p] == 0) mn = B.h[p] / (T.cs + sqrt(vx * vx + vy * vy + vz * vz));
     }
     px = wave_sum(px);
     py = wave_sum(py);
@@ -234,6 +228,40 @@ __global__ __launch_bounds__(kBS) void k_prepare(const TubeConst* __restrict__ t
         sh[1][w] = py;
         sh[2][w] = pz;
         sh[3][w] = E;
+        sh[4][w] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a[4] = {0, 0, 0, 0}, m2 = 1e300;
+        for (int k = 0; k < kBS / 64; ++k) {
+            for (int c = 0; c < 4; ++c) a[c] += sh[c][k];
+            m2 = fmin(m2, sh[4][k]);
+        }
+        double* o = part + 5ll * blockIdx.x;
+        for (int c = 0; c < 4; ++c) o[c] = a[c];
+        o[4] = m2;
+    }
+}
+
+__global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                 const double* __restrict__ part) {
+    __shared__ double sh[5][kBS / 64];
+    const int t = blockIdx.x;
+    const TubeConst& T = tc[t];
+    double mn = 1e300, a4[4] = {0, 0, 0, 0};
+    for (int s = 0; s < T.nsp; ++s) {
+        const int sg = T.pseg[s];
+        for (int b = mp.blk[sg] + threadIdx.x; b < mp.blk[sg + 1]; b += kBS) {
+            const double* o = part + 5ll * b;
+            for (int c = 0; c < 4; ++c) a4[c] += o[c];
+            mn = fmin(mn, o[4]);
+        }
+    }
+    for (int c = 0; c < 4; ++c) a4[c] = wave_sum(a4[c]);
+    mn = wave_min(mn);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        for (int c = 0; c < 4; ++c) sh[c][w] = a4[c];
         sh[4][w] = mn;
     }
     __syncthreads();
@@ -386,7 +414,10 @@ __global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__
 }
 
 // Final position = cell start + number of cell members with a smaller id.
-__global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const int* __restrict__ cur, PBuf B,
+// A finished tube is still sorted every step (its positions moved on its last drift, so
+// the order can change once); its last acceleration and de/dt move with it.
+__global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __restrict__ td,
+                                                      const int* __restrict__ cur, PBuf B,
                                                       const int* __restrict__ start, const int* __restrict__ tsrc,
                                                       const int* __restrict__ tid, const int* __restrict__ tkey,
                                                       int* __restrict__ wkey) {
@@ -417,6 +448,12 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const int* __restr
     if (ip.sg.species == 0) {
         B.e[wo + dst] = B.e[so + src];
         B.p[wo + dst] = B.p[so + src];
+        if (td[ip.sg.tube].done) {
+            B.ax[wo + dst] = B.ax[so + src];
+            B.ay[wo + dst] = B.ay[so + src];
+            B.az[wo + dst] = B.az[so + src];
+            B.ed[wo + dst] = B.ed[so + src];
+        }
     } else {
         B.ts[wo + dst] = B.ts[so + src];
     }
@@ -1268,8 +1305,9 @@ int scan_blocks(int n) { return (n + kScanTile - 1) / kScanTile; }
 
 size_t mask_words(int gas_chunks) { return (size_t)gas_chunks * kMaskSlots * 64; }
 
-hipError_t launch_prepare(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_prepare, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.B);
+hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s) {
+    hipLaunchKernelGGL(k_prepare_part, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.B, scratch);
+    hipLaunchKernelGGL(k_prepare, dim3(a.ntubes), dim3(kBS), 0, s, a.pmap, a.tc, a.td, scratch);
     return hipGetLastError();
 }
 
@@ -1299,7 +1337,8 @@ hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.start, a.tsrc,
+    hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.cur, a.B, a.start,
+                       a.tsrc,
                        a.tid, a.tkey, a.wkey);
     return hipGetLastError();
 }

@@ -36,6 +36,7 @@ struct TubeConst {
     int cell_off[kMaxSpecies];
     int p_off[kMaxSpecies];
     int np[kMaxSpecies];
+    int pseg[kMaxSpecies];   // index of segment (species, tube) in the particle map, -1 if none
     // physics
     double gamma, cs, alpha, beta, eps, eta;
     double K[kMaxDust], mu[kMaxDust], eps_fb[kMaxDust], t_fb;

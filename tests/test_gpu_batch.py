@@ -1,0 +1,107 @@
+"""Batching and serving invariance on the GPU: a tube's result does not depend on the
+other tubes in its batch, nor on being moved between batches by the server.  Every
+per-tube reduction runs in a fixed order inside the tube, so the results are bit-identical."""
+import numpy as np
+import pytest
+
+from dustyshock3d_amd import Simulation, abi, configs
+from dustyshock3d_amd.batch import Batch, Server
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "h", "e", "t_stop", "ax", "ay", "az")
+
+
+def small_specs(precision):
+    # uneven sizes, drag and dust loading, from the sweep distribution but short tubes
+    out = []
+    for i, (w, dx, K, d2g) in enumerate([(0.03, 0.01, 0.05, 0.3), (0.05, 0.006, 200.0, 2.0),
+                                         (0.02, 0.008, 3.0, 8.0), (0.04, 0.005, 1000.0, 0.05)]):
+        out.append(configs.tube(w, dx, K, d2g, t_end=0.02, dt_mode=abi.DT_CFL, precision=precision, tube_index=i))
+    return out
+
+
+def alone(spec, steps):
+    s = Simulation.from_spec(spec)
+    s.step(steps)
+    return s
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_batch_matches_alone_bit_exact(precision):
+    specs = small_specs(precision)
+    print(f"\nbatch precision={precision} tubes={len(specs)}", flush=True)
+    b = Batch(specs)
+    b.run(400)
+    for i, spec in enumerate(specs):
+        st = b.status(i)
+        assert st["done"] and abs(st["t"] - 0.02) < 1e-12
+        s = alone(spec, 400)
+        assert s.status()["steps"] == st["steps"]
+        for sp in (0, 1):
+            rb, rs = b.download(i, sp), s.download(sp)
+            for k in FIELDS:
+                assert np.array_equal(rb[k], rs[k]), (i, sp, k)
+        assert np.array_equal(b.conservation(i), s.conservation())
+
+
+def test_server_results_match_alone():
+    specs = small_specs(abi.MIXED)
+    sv = Server(hbm_budget=64e9, steps_per_epoch=7)
+    ids = [sv.submit(specs[0], 0.0), sv.submit(specs[1], 0.0)]
+    sv.pump()
+    sv.pump()
+    ids += [sv.submit(specs[2], 1.0), sv.submit(specs[3], 1.0)]   # join a running batch
+    for _ in range(200):
+        r, q = sv.pump()
+        if r == 0 and q == 0:
+            break
+    st = sv.stats()
+    print(f"\nserver rebuilds={st['rebuilds']} epochs={st['epochs']}", flush=True)
+    assert st["rebuilds"] >= 2
+    for rid, spec in zip(ids, specs):
+        p = sv.poll(rid)
+        assert p["state"] == abi.REQ_DONE
+        s = alone(spec, 400)
+        for sp in (0, 1):
+            rv, rs = sv.result(rid, sp), s.download(sp)
+            for k in FIELDS:
+                assert np.array_equal(rv[k], rs[k]), (rid, sp, k)
+
+
+def _raw_batch(arrays_per_tube, params):
+    import ctypes as C
+    from dustyshock3d_amd.abi import Params, make_view
+    lib = abi.lib()
+    h = C.c_void_p()
+    assert lib.dsh_batch_create((Params * len(params))(*params), len(params), C.byref(h)) == abi.OK
+    for t, per_species in enumerate(arrays_per_tube):
+        for s, a in enumerate(per_species):
+            v = make_view({k: a[k] for k in ("x", "y", "z", "vx", "vy", "vz", "m", "rho", "p", "e", "h",
+                                             "t_stop", "ind")})
+            assert lib.dsh_batch_upload(h, t, s, C.byref(v), len(a["x"]) - 1) == abi.OK
+    return lib, h
+
+
+def test_failed_tube_fails_alone():
+    """A tube whose smoothing length outgrows its grid's search window stops with an error
+    before any pair kernel runs for it; the other tube of the batch completes."""
+    import ctypes as C
+    from dustyshock3d_amd.sim import tube_ic, tube_params
+    good = configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=5, tube_index=1)
+    bad = configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=5, tube_index=2)
+    arrays = [[tube_ic(good, 0), tube_ic(good, 1)], [tube_ic(bad, 0), tube_ic(bad, 1)]]
+    # the grid comes from the smallest initial h; one particle with 200x that h needs a
+    # search window of ~190 cells, above the 64-cell limit
+    i = len(arrays[1][0]["h"]) // 2
+    arrays[1][0]["h"][i] = 200.0 * arrays[1][0]["h"].min()
+    lib, h = _raw_batch(arrays, [tube_params(good), tube_params(bad)])
+    nd = C.c_int()
+    assert lib.dsh_batch_run(h, 5, C.byref(nd)) == abi.ERR_GRID
+    assert b"tube 1" in lib.dsh_batch_last_error(h)
+    steps, done = C.c_int(), C.c_int()
+    lib.dsh_batch_status(h, 0, None, None, C.byref(steps), C.byref(done))
+    assert (steps.value, done.value) == (5, 1)
+    lib.dsh_batch_status(h, 1, None, None, C.byref(steps), C.byref(done))
+    assert (steps.value, done.value) == (0, 1)
+    lib.dsh_batch_destroy(h)
