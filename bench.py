@@ -52,6 +52,28 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
     return 0
 
 
+# kernels behind each pass, as named in the rocprofv3 summaries under profiles/
+PASS_KERNELS = {
+    "density": ["k_pairs_tiled<float, 0, false>"],
+    "force": ["k_pairs_tiled<float, 1, false>"],
+    "idic": ["k_idic"],
+    "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
+             "k_cellmax", "k_radius"],
+}
+
+
+def measured_traffic(workload, stage):
+    """HBM bytes per launch of a pass from the committed FETCH_SIZE/WRITE_SIZE summary (or None)."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    kern = json.load(open(path))["kernels"]
+    names = PASS_KERNELS.get(stage, [])
+    if not names or any(n not in kern for n in names):
+        return None
+    return sum(kern[n]["read_bytes"] + kern[n]["write_bytes"] for n in names)
+
+
 def cpu_baseline(spec, threads, steps=1):
     """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube."""
     import oracle
@@ -146,7 +168,9 @@ def main():
                    "dt": st1["dt"], "setup_s": round(setup_s, 2),
                    "pair_kernels": "tiled" if args.variant == 1 else "direct"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(args.config, dominant) if args.variant == 1 else None,
+                     "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms},
         "step_roofline": {"algo_bytes": step_bytes, "event_ms": step_ms,
                           "frac": step_bytes / (wall / max(steps_done, 1)) / 1e9 / HBM_PEAK_GBS},
@@ -294,10 +318,30 @@ def serve_mode(args):
     print(json.dumps(out))
 
 
+def copy_mode(args):
+    """A plain device-to-device copy of known size: the reference for scaling the HBM
+    counters (FETCH_SIZE / WRITE_SIZE) of the step kernels to bytes."""
+    import torch
+
+    n = int(args.copy_bytes) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.copy_reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.copy_reps
+    print(json.dumps({"metric": "device copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "reps": args.copy_reps, "GBs": 8 * n / dt / 1e9}))
+
+
 if __name__ == "__main__":
     if "--mode" in sys.argv:
         ap = argparse.ArgumentParser()
-        ap.add_argument("--mode", choices=["sweep", "serve"], required=True)
+        ap.add_argument("--mode", choices=["sweep", "serve", "copy"], required=True)
+        ap.add_argument("--copy-bytes", type=float, default=1 << 30)
+        ap.add_argument("--copy-reps", type=int, default=5)
         ap.add_argument("--gpus", type=int, default=1)
         ap.add_argument("--variant", type=int, default=1)
         ap.add_argument("--tubes", type=int, default=512)
@@ -311,6 +355,6 @@ if __name__ == "__main__":
         ap.add_argument("--seed", type=int, default=7)
         a = ap.parse_args()
         import numpy as np  # noqa: F811
-        (sweep_mode if a.mode == "sweep" else serve_mode)(a)
+        {"sweep": sweep_mode, "serve": serve_mode, "copy": copy_mode}[a.mode](a)
     else:
         main()
