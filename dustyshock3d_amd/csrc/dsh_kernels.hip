@@ -95,7 +95,8 @@ template <> __device__ __forceinline__ float rcp<float>(float x) { return __buil
 
 template <typename R> __device__ __forceinline__ R sqrt_r(R x);
 template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return sqrt(x); }
-template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_sqrtf(x); }
+// v_sqrt_f32 (1 ulp); arguments are squared distances, never denormal-small in practice.
+template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // Plain sum in FP64, Kahan-compensated sum in FP32.
 template <typename R> struct Acc;
@@ -117,21 +118,18 @@ template <> struct Acc<float> {
 
 // Cubic spline W(r,h) (kernel.cu:11-22) and dW/dr for r >= 0 (kernel.cu:24-38),
 // written with one reciprocal of h per pair.
+// Both branches are evaluated and one is selected, so a wave with mixed q runs one path.
 template <typename R> __device__ __forceinline__ R spline_w(R q) {
-    if (q < R(1)) return R(1) - R(1.5) * q * q + R(0.75) * q * q * q;
-    if (q <= R(2)) {
-        const R u = R(2) - q;
-        return R(0.25) * u * u * u;
-    }
-    return R(0);
+    const R inner = R(1) - R(1.5) * q * q + R(0.75) * q * q * q;
+    const R u = R(2) - q;
+    const R outer = R(0.25) * u * u * u;
+    return q < R(1) ? inner : (q <= R(2) ? outer : R(0));
 }
 template <typename R> __device__ __forceinline__ R spline_dw(R q) {
-    if (q <= R(1)) return R(-3) * q + R(2.25) * q * q;
-    if (q <= R(2)) {
-        const R u = R(2) - q;
-        return R(-0.75) * u * u;
-    }
-    return R(0);
+    const R inner = R(-3) * q + R(2.25) * q * q;
+    const R u = R(2) - q;
+    const R outer = R(-0.75) * u * u;
+    return q <= R(1) ? inner : (q <= R(2) ? outer : R(0));
 }
 
 // ---------------------------------------------------------------- neighbour traversal
@@ -727,9 +725,37 @@ __device__ __forceinline__ int wave_max_int(int v) {
     return v;
 }
 
+// One candidate of the FP64 neighbour predicate, as a bit at position t (t < 32, uniform).
+// One candidate's position and smoothing length, gathered from the staged arrays.
+struct P4 {
+    double x, y, z, h;
+};
+// Pair predicate d2 < (h_i + h_j)^2 with d2 = fma(dz, dz, fma(dy, dy, dx*dx)); a periodic
+// image offset is added to the coordinate difference, not to the coordinate.
+__device__ __forceinline__ uint32_t pred1(const P4& c, double xi, double yi, double zi, double hi) {
+    const double dx = c.x - xi;
+    const double dy = c.y - yi;
+    const double dz = c.z - zi;
+    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double hs = hi + c.h;
+    return dd < hs * hs ? 1u : 0u;
+}
+__device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double yi, double zi, double hi, double shy,
+                                                double shz) {
+    const double dx = c.x - xi;
+    const double dy = (c.y - yi) + shy;
+    const double dz = (c.z - zi) + shz;
+    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+    const double hs = hi + c.h;
+    return dd < hs * hs ? 1u : 0u;
+}
+
+// The position arrays carry one window of padding: a window reads base + t, t < 64, with
+// base clamped to the staged count, and bits past a lane's own range are masked.  Separate
+// arrays (not one record per candidate) keep the lanes' 8-byte reads on distinct banks.
 template <typename R, int KIND, bool kStats>
 struct TileLds {
-    double x[kTileP], y[kTileP], z[kTileP], h[kTileP];
+    double x[kTileP + 64], y[kTileP + 64], z[kTileP + 64], h[kTileP + 64];
     double vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
     R m[kTileP];
     R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
@@ -890,43 +916,46 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                         unsigned long long mask = 0ull;
                         if (KIND == 1 && use_masks && slot < kMaskSlots) {
                             mask = mrow[(size_t)slot * 64];
-                        } else if (!shifted) {
-                            for (int t = 0; t < wn; ++t) {
-                                const int u = a + w0 + t;
-                                if (u < e) {
-                                    const double dx = L.x[u] - xi;
-                                    const double dy = L.y[u] - yi;
-                                    const double dz = L.z[u] - zi;
-                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                                    const double hs = hi + L.h[u];
-                                    if (dd < hs * hs) mask |= 1ull << t;
-                                }
-                            }
                         } else {
-                            for (int t = 0; t < wn; ++t) {
-                                const int u = a + w0 + t;
-                                if (u < e) {
-                                    const double dx = L.x[u] - xi;
-                                    const double dy = (L.y[u] - yi) + shy;
-                                    const double dz = (L.z[u] - zi) + shz;
-                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                                    const double hs = hi + L.h[u];
-                                    if (dd < hs * hs) mask |= 1ull << t;
-                                }
+                            // uniform trip count: every lane evaluates wn candidates (index
+                            // clamped into the staged piece); bits past the lane's own range
+                            // are cleared once per window
+                            const int rem = len - w0;
+                            const unsigned long long valid =
+                                rem >= 64 ? ~0ull : (rem <= 0 ? 0ull : ((1ull << rem) - 1ull));
+                            const int base = min(a + w0, pn);
+                            // candidates are shifted in from the top, so candidate t lands on bit t
+                            const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
+                            uint32_t mlo = 0u, mhi = 0u;
+                            const int n0 = min(wn, 32);
+                            if (!shifted) {
+#pragma unroll 4
+                                for (int t = wn - 1; t >= 32; --t) mhi = (mhi << 1) | pred1(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi);
+#pragma unroll 4
+                                for (int t = n0 - 1; t >= 0; --t) mlo = (mlo << 1) | pred1(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi);
+                            } else {
+#pragma unroll 4
+                                for (int t = wn - 1; t >= 32; --t)
+                                    mhi = (mhi << 1) | pred1_shift(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi, shy, shz);
+#pragma unroll 4
+                                for (int t = n0 - 1; t >= 0; --t)
+                                    mlo = (mlo << 1) | pred1_shift(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi, shy, shz);
                             }
+                            mask = (((unsigned long long)mhi << 32) | mlo) & valid;
                         }
                         if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
                         while (mask) {
                             const int t = __ffsll((long long)mask) - 1;
                             mask &= mask - 1ull;
                             const int u = a + w0 + t;
-                            const double dx = L.x[u] - xi;
-                            const double dy = (L.y[u] - yi) + shy;
-                            const double dz = (L.z[u] - zi) + shz;
+                            const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
+                            const double dx = c.x - xi;
+                            const double dy = (c.y - yi) + shy;
+                            const double dz = (c.z - zi) + shz;
                             const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
                             if (KIND == 0) {
                                 const R r = sqrt_r<R>((R)dd);
-                                const R hh = (R)(0.5 * (hi + L.h[u]));
+                                const R hh = (R)(0.5 * (hi + c.h));
                                 const R ih = rcp<R>(hh);
                                 const R w = (R)kInvPi * (ih * ih * ih) * spline_w<R>(r * ih);
                                 a0.add(L.m[u] * w);
@@ -935,7 +964,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                     hash += pair_tag(L.id[u], sy, sz);
                                 }
                             } else if (dd > 0.0) {
-                                const double hhd = 0.5 * (hi + L.h[u]);
+                                const double hhd = 0.5 * (hi + c.h);
                                 if (sizeof(R) == 4 && L.ind[u] != 0) {
                                     // a frozen neighbour has no partner term: FP64 keeps the
                                     // momentum it injects exact to rounding
