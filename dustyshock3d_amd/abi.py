@@ -160,6 +160,12 @@ def _declare(lib):
         "dsh_tube_count": (C.c_int, [P(TubeSpec), _IP]),
         "dsh_tube_ic": (C.c_int, [P(TubeSpec), C.c_int, P(SoaView)]),
         "dsh_read_init": (C.c_int, [C.c_char_p, P(Params)]),
+        "dsh_reference_setup": (C.c_int, [P(Params), P(C.c_int)]),
+        "dsh_reference_ic": (C.c_int, [P(Params), C.c_int, P(SoaView)]),
+        "dsh_write_data": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     P(SoaView), C.c_int]),
+        "dsh_reference_nn": (C.c_int, [P(Params), P(C.c_double), P(C.c_double), P(C.c_double), C.c_int,
+                                       P(C.c_int)]),
         "dsh_create": (C.c_int, [P(Params), P(vp)]),
         "dsh_upload_species": (C.c_int, [vp, C.c_int, P(SoaView), C.c_int]),
         "dsh_step": (C.c_int, [vp, C.c_int, P(C.c_double)]),

@@ -163,9 +163,13 @@ int Engine::prepare() {
         const dsh_params& P = params_[t];
         if (P.precision != precision_) return fail(DSH_ERR_ARG, "tubes of one batch must share the precision");
         if (P.n_dust < 1 || P.n_dust > kMaxDust) return fail(DSH_ERR_ARG, "n_dust must be 1 or 2");
-        if (P.drag_mode != 1) return fail(DSH_ERR_UNSUPPORTED, "only drag_mode 1 (t_stop = rho_d/K) is built");
+        if (P.drag_mode != 0 && P.drag_mode != 1) return fail(DSH_ERR_ARG, "drag_mode must be 0 or 1");
         for (int s = 0; s <= P.n_dust; ++s)
             if (!staged_[(size_t)t * kMaxSpecies + s].set) return fail(DSH_ERR_STATE, "species not uploaded");
+        if (P.drag_mode == 0)
+            for (int s = 1; s <= P.n_dust; ++s)
+                if (staged_[(size_t)t * kMaxSpecies + s].n != staged_[(size_t)t * kMaxSpecies].n)
+                    return fail(DSH_ERR_ARG, "drag_mode 0 pairs dust index i with gas index i: counts must match");
     }
     tc_h_.assign(T, TubeConst{});
     for (int t = 0; t < T; ++t) {
@@ -192,7 +196,9 @@ int Engine::prepare() {
             C.K[k] = P.K[k];
             C.mu[k] = P.mass_ratio[k];
             C.eps_fb[k] = P.eps_fallback[k];
+            C.R[k] = P.R_dust[k];
         }
+        C.drag_mode = P.drag_mode;
         C.t_fb = P.tstop_fallback;
         C.tau = P.tau;
         C.cfl = P.cfl;
@@ -272,6 +278,14 @@ int Engine::prepare() {
     if ((rc = make_map(gseg_, 64, a_.gmap64))) return rc;
     a_.variant = pair_variant_;
     if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
+    a_.ginv = nullptr;
+    for (int t = 0; t < T; ++t)
+        if (tc_h_[t].drag_mode == 0) {
+            long long ngas = 0;
+            for (const Seg& g : gseg_) ngas += g.n;
+            if ((rc = alloc(&a_.ginv, (size_t)ngas))) return rc;
+            break;
+        }
 
     // particle buffers (two copies of each persistent field)
     PBuf& B = a_.B;

@@ -70,6 +70,67 @@ bool spec_ok(const dsh_tube_spec* t) {
 
 }  // namespace
 
+// ---- the reference's own problem: init_Sod_X_Dust (kernel.cu:1391-1575) ----
+//
+// Gas on two lattices of spacing dlh = (m / rho)^(1/3) = 1 / Particle_on_length, x <= 0
+// anchored at 0 going -x and x > 0 starting at +dlh, kept where |x| <= 1, |y|, |z| <= 0.8;
+// e = 2.5 left, 2.0 right, rho = 1, p from the equation of state with gamma 1.4 and c = 1.
+// The frozen flag is evaluated for every lattice candidate and stored on the last placed
+// particle (kernel.cu:1441-1448,1481-1488), with X_cube standing in for Y_cube in the y
+// test; before the first placement the reference writes Ind_gas[-1], which is skipped.
+// Two dust species copy the gas positions and flags, with mass and density scaled by 0.75
+// and 0.5, and t_stop = R / (sqrt(1.4 p / rho) rho) with R = 0.01 and 1.0.
+namespace {
+
+struct RefLattice {
+    double dlh[2], x0[2], sgn[2];
+    int im[2], jm[2], km[2];
+    double e[2], p[2], rho, m;
+};
+
+RefLattice ref_lattice(const dsh_params& P) {
+    RefLattice L{};
+    L.m = 1.0 / (P.particles_per_length * P.particles_per_length * P.particles_per_length);
+    L.rho = 1.0;
+    L.e[0] = 2.5;
+    L.e[1] = 2.0;
+    for (int s = 0; s < 2; ++s) {
+        L.p[s] = P.eos_type == 0 ? L.rho * L.e[s] * (1.4 - 1) : 1.0 * 1.0 * L.rho;
+        L.dlh[s] = std::pow(L.m / L.rho, 1 / 3.0);
+        const double span = s == 0 ? (0.0 - P.x_min) : (P.x_max - 0.0);
+        L.im[s] = (int)(span / L.dlh[s]) + 1;
+        L.jm[s] = (int)((P.y_max - P.y_min) / L.dlh[s]) + 1;
+        L.km[s] = (int)((P.z_max - P.z_min) / L.dlh[s]) + 1;
+    }
+    return L;
+}
+
+// Visits the lattice candidates in the reference's loop order; place(x, y, z, side) is
+// called for placed particles and flag(ind) after every candidate.
+template <class Place, class Flag>
+void ref_walk(const dsh_params& P, const RefLattice& L, Place&& place, Flag&& flag) {
+    const double border = 2.5 * P.h;
+    const double X_cube = 1.0, Y_cube = 0.8, Z_cube = 0.8;
+    for (int side = 0; side < 2; ++side) {
+        const double d = L.dlh[side];
+        for (int i = 0; i <= L.im[side]; i++)
+            for (int j = 0; j <= L.jm[side]; j++)
+                for (int k = 0; k <= L.km[side]; k++) {
+                    const double x = side == 0 ? 0.0 - (double)(i * d) : d + (double)(i * d);
+                    const double y = P.y_min + (double)(j * d);
+                    const double z = P.z_min + (double)(k * d);
+                    if ((std::fabs(x) <= X_cube) && (std::fabs(y) <= Y_cube) && (std::fabs(z) <= Z_cube))
+                        place(x, y, z, side);
+                    flag(((std::fabs(x) <= X_cube - border) && (std::fabs(y) <= X_cube - border) &&
+                          (std::fabs(z) <= Z_cube - border))
+                             ? 0
+                             : 1);
+                }
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
 int dsh_struct_sizes(size_t out[4]) {
@@ -107,7 +168,7 @@ int dsh_default_params(dsh_params* p) {
     p->h_mode = 0;
     p->eta = 1.2;
     p->n_dust = 2;
-    p->drag_mode = 1;
+    p->drag_mode = 0;
     p->K[0] = p->K[1] = 1.0;
     p->R_dust[0] = 0.01;
     p->R_dust[1] = 1.0;
@@ -148,6 +209,7 @@ int dsh_tube_params(const dsh_tube_spec* t, dsh_params* p) {
     p->periodic_z = 1;
     p->h_mode = 1;
     p->n_dust = 1;
+    p->drag_mode = 1;
     p->K[0] = p->K[1] = t->K;
     p->mass_ratio[0] = p->mass_ratio[1] = t->dust_to_gas;
     p->eps2_uses_count1 = 0;
@@ -213,6 +275,65 @@ int dsh_tube_ic(const dsh_tube_spec* t, int species, const dsh_soa_view* v) {
                     if (v->cell) v->cell[id] = -1;
                 }
     }
+    return DSH_OK;
+}
+
+int dsh_reference_setup(dsh_params* p, int* n_per_species) {
+    if (!p || !n_per_species) return DSH_ERR_ARG;
+    if (!(p->particles_per_length > 0.0) || !(p->h > 0.0)) return DSH_ERR_ARG;
+    // parameter overrides of init_Sod_X_Dust (kernel.cu:1495-1502) and the averaging grid
+    // built from them (kernel.cu:1577-1582): the averaging cell is the neighbour cell here
+    p->n_dust = 2;
+    p->avg_cell_coeff = 0.5;
+    p->mass_ratio[0] = 0.75;
+    p->mass_ratio[1] = 0.5;
+    p->R_dust[0] = 0.01;
+    p->R_dust[1] = 1.0;
+    p->cell_size = p->avg_cell_coeff * p->h;
+    p->avg_factor = 1;
+    const RefLattice L = ref_lattice(*p);
+    long long n = 0;
+    ref_walk(*p, L, [&](double, double, double, int) { ++n; }, [](int) {});
+    if (n == 0 || n > 2000000000ll) return DSH_ERR_ARG;
+    if (p->max_particles > 0 && n > p->max_particles) return DSH_ERR_ARG;
+    for (int s = 0; s < DSH_MAX_SPECIES; ++s) n_per_species[s] = (int)n;
+    return DSH_OK;
+}
+
+int dsh_reference_ic(const dsh_params* p, int species, const dsh_soa_view* v) {
+    if (!p || !v || species < 0 || species > 2) return DSH_ERR_ARG;
+    const RefLattice L = ref_lattice(*p);
+    const double ratio = species == 0 ? 1.0 : p->mass_ratio[species - 1];
+    int last = -1;
+    ref_walk(
+        *p, L,
+        [&](double x, double y, double z, int side) {
+            const int q = ++last;
+            if (v->x) v->x[q] = x;
+            if (v->y) v->y[q] = y;
+            if (v->z) v->z[q] = z;
+            if (v->vx) v->vx[q] = 0.0;
+            if (v->vy) v->vy[q] = 0.0;
+            if (v->vz) v->vz[q] = 0.0;
+            if (v->ax) v->ax[q] = 0.0;
+            if (v->ay) v->ay[q] = 0.0;
+            if (v->az) v->az[q] = 0.0;
+            if (v->m) v->m[q] = L.m * ratio;
+            if (v->rho) v->rho[q] = L.rho * ratio;
+            if (v->p) v->p[q] = species == 0 ? L.p[side] : 0.0;
+            if (v->e) v->e[q] = species == 0 ? L.e[side] : 0.0;
+            if (v->h) v->h[q] = p->h;
+            if (v->t_stop) {
+                const double R = species == 0 ? 0.0 : p->R_dust[species - 1];
+                v->t_stop[q] = species == 0 ? 0.0 : R / (std::sqrt(p->gamma * L.p[side] / L.rho) * L.rho);
+            }
+            if (v->edot) v->edot[q] = 0.0;
+            if (v->nn) v->nn[q] = -1;
+            if (v->cell) v->cell[q] = -1;
+        },
+        [&](int ind) {
+            if (last >= 0 && v->ind) v->ind[last] = ind;
+        });
     return DSH_OK;
 }
 

@@ -44,6 +44,7 @@ struct StepArgs {
     Map pmap, gmap, cmap, amap;
     Map pmap64, gmap64;    // 64 targets per block (tiled pair kernels)
     unsigned long long* masks;   // gas density hit masks reused by the force pass
+    int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
     int variant;
 };
 

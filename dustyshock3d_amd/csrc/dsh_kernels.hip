@@ -40,8 +40,10 @@ __device__ __forceinline__ int find_seg(const Map& mp, int b) {
 
 __device__ __forceinline__ int floordiv(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
 
-__device__ __forceinline__ int axis_cell(double x, double o, double ic, int n) {
-    const double t = (x - o) * ic;
+// Cell index along one axis: int((x - o) / c) as kernel.cu:1949 and IDIC.h:28 compute it
+// (FP64 divide, truncation), clamped into [0, n).
+__device__ __forceinline__ int axis_cell(double x, double o, double c, int n) {
+    const double t = (x - o) / c;
     if (!(t >= 0.0)) return 0;
     if (t >= (double)n) return n - 1;
     return (int)t;
@@ -305,9 +307,9 @@ __global__ __launch_bounds__(kBS) void k_key(Map mp, const TubeConst* __restrict
     const long long so = (long long)(*cur) * B.ntot;
     int k = -1;
     if (valid) {
-        const int kx = axis_cell(B.x[so + p], T.o[0], T.ic[0], T.n[0]);
-        const int ky = axis_cell(B.y[so + p], T.o[1], T.ic[1], T.n[1]);
-        const int kz = axis_cell(B.z[so + p], T.o[2], T.ic[2], T.n[2]);
+        const int kx = axis_cell(B.x[so + p], T.o[0], T.c[0], T.n[0]);
+        const int ky = axis_cell(B.y[so + p], T.o[1], T.c[1], T.n[1]);
+        const int kz = axis_cell(B.z[so + p], T.o[2], T.c[2], T.n[2]);
         k = T.cell_off[ip.sg.species] + kx + T.n[0] * (ky + T.n[1] * kz);
     }
     const int lane = threadIdx.x & 63;
@@ -418,7 +420,7 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
                                                       const int* __restrict__ cur, PBuf B,
                                                       const int* __restrict__ start, const int* __restrict__ tsrc,
                                                       const int* __restrict__ tid, const int* __restrict__ tkey,
-                                                      int* __restrict__ wkey) {
+                                                      int* __restrict__ wkey, int* __restrict__ ginv) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
@@ -444,6 +446,7 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
     B.ind[wo + dst] = B.ind[so + src];
     B.id[wo + dst] = my;
     if (ip.sg.species == 0) {
+        if (ginv) ginv[ip.sg.off + my] = dst;
         B.e[wo + dst] = B.e[so + src];
         B.p[wo + dst] = B.p[so + src];
         if (td[ip.sg.tube].done) {
@@ -599,7 +602,7 @@ __global__ __launch_bounds__(kBS) void k_density(Map mp, const TubeConst* __rest
             const double p = (T.eos == 0) ? rho * B.e[wo + q] * (T.gamma - 1) : T.cs * T.cs * rho;
             B.p[wo + q] = p;
         } else {
-            B.ts[wo + q] = rho / T.K[s - 1];
+            if (T.drag_mode == 1) B.ts[wo + q] = rho / T.K[s - 1];
         }
         if (kStats) {
             ncount[q] = f.cnt;
@@ -1007,7 +1010,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
             if (s == 0) {
                 B.p[wo + q] = (T.eos == 0) ? rho * B.e[wo + q] * (T.gamma - 1) : T.cs * T.cs * rho;
             } else {
-                B.ts[wo + q] = rho / T.K[s - 1];
+                if (T.drag_mode == 1) B.ts[wo + q] = rho / T.K[s - 1];
             }
         }
         if (kStats) {
@@ -1092,7 +1095,8 @@ __device__ __forceinline__ void idic_solve(int nd, double vg, const double* vd, 
 // and the cell's conservation / CFL partials.
 __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restrict__ tc,
                                               const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
-                                              const int* __restrict__ start, double* __restrict__ part) {
+                                              const int* __restrict__ start, const int* __restrict__ ginv,
+                                              double* __restrict__ part) {
     const ItemPos ip = locate(mp, kBS / 64);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int la = ip.li + wave;
@@ -1133,7 +1137,16 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
                     sd[s - 1][0] += B.vx[wo + k];
                     sd[s - 1][1] += B.vy[wo + k];
                     sd[s - 1][2] += B.vz[wo + k];
-                    sd[s - 1][3] += B.ts[wo + k];
+                    double ts = B.ts[wo + k];
+                    if (T.drag_mode == 0 && B.ind[wo + k] == 0) {
+                        // t_stop_Kernel_IDIC (IDIC.h:78-91): the gas particle with the same
+                        // index, this step's pressure and density
+                        const long long gq = wo + ginv[T.p_off[0] + B.id[wo + k]];
+                        const double rg = B.rho[gq];
+                        ts = T.R[s - 1] / (sqrt(T.gamma * B.p[gq] / rg) * rg);
+                        B.ts[wo + k] = ts;
+                    }
+                    sd[s - 1][3] += ts;
                 }
             }
         }
@@ -1367,8 +1380,7 @@ hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
 
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.cur, a.B, a.start,
-                       a.tsrc,
-                       a.tid, a.tkey, a.wkey);
+                       a.tsrc, a.tid, a.tkey, a.wkey, a.ginv);
     return hipGetLastError();
 }
 
@@ -1440,7 +1452,7 @@ hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
 
 hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
-                       a.part);
+                       a.ginv, a.part);
     return hipGetLastError();
 }
 

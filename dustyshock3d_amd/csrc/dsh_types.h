@@ -40,6 +40,8 @@ struct TubeConst {
     // physics
     double gamma, cs, alpha, beta, eps, eta;
     double K[kMaxDust], mu[kMaxDust], eps_fb[kMaxDust], t_fb;
+    double R[kMaxDust];    // drag_mode 0: t_stop = R / (sqrt(gamma p_g / rho_g) rho_g)
+    int drag_mode;         // 0: reference law with the gas particle of the same index, 1: rho_d / K
     double tau, cfl, t_end;
     int eos, h_mode, eps2_quirk, dt_mode, max_steps, precision;
 };

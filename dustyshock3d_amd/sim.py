@@ -53,6 +53,42 @@ def read_init(path: str) -> Params:
     return p
 
 
+def reference_setup(p: Params) -> int:
+    """Apply init_Sod_X_Dust's parameter overrides to p; particles per species."""
+    n = (C.c_int * 3)()
+    _check(abi.lib().dsh_reference_setup(C.byref(p), n), "reference_setup")
+    return int(n[0])
+
+
+def reference_ic(p: Params, species: int) -> dict:
+    """init_Sod_X_Dust placement of one species (kernel.cu:1391-1575), index order."""
+    q = Params.from_buffer_copy(p)
+    n = reference_setup(q)
+    arrs = empty_arrays(n)
+    v = make_view(arrs)
+    _check(abi.lib().dsh_reference_ic(C.byref(p), species, C.byref(v)), "reference_ic")
+    return arrs
+
+
+def write_data(directory: str, num: int, t: float, tau: float, h: float, species: int, arrays: dict):
+    """Data_out / Data_out_dust files of one species (kernel.cu:232-826)."""
+    n = len(arrays["x"])
+    v = make_view(arrays)
+    _check(abi.lib().dsh_write_data(directory.encode(), num, t, tau, h, species, C.byref(v), n - 1),
+           "write_data")
+
+
+def reference_nn(p: Params, x, y, z) -> np.ndarray:
+    """Next-particle pointers of the reference's lists on its 2h grid."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    nn = np.empty(len(x), dtype=np.int32)
+    dp = C.POINTER(C.c_double)
+    _check(abi.lib().dsh_reference_nn(C.byref(p), x.ctypes.data_as(dp), y.ctypes.data_as(dp),
+                                      z.ctypes.data_as(dp), len(x) - 1, nn.ctypes.data_as(C.POINTER(C.c_int))),
+           "reference_nn")
+    return nn
+
+
 UPLOAD_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "m", "rho", "p", "e", "h", "t_stop", "ind")
 
 

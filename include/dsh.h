@@ -128,8 +128,27 @@ int dsh_default_params(dsh_params* out);
 int dsh_tube_params(const dsh_tube_spec* spec, dsh_params* out);
 int dsh_tube_count(const dsh_tube_spec* spec, int* n_per_species);
 int dsh_tube_ic(const dsh_tube_spec* spec, int species, const dsh_soa_view* out);
-/* Init.txt reader (kernel.cu:1705-1742); extension lines after line 18 are optional. */
+/* Init.txt reader (kernel.cu:1705-1742); extension lines after line 18 are optional.
+   Without them the parameters are the reference run's: two dust species with the
+   reference stopping-time law (drag_mode 0), global h, fixed tau. */
 int dsh_read_init(const char* path, dsh_params* out);
+/* The reference's own problem, init_Sod_X_Dust (kernel.cu:1391-1575): applies the parameter
+   overrides of that function to *p (two dust species, mass ratios 0.75 / 0.5, R_dust 0.01 / 1,
+   averaging cell 0.5 h, kernel.cu:1495-1502) and returns the particle count of each species. */
+int dsh_reference_setup(dsh_params* p, int* n_per_species);
+/* Particle placement of init_Sod_X_Dust for one species (0 gas, 1..2 dust), index order. */
+int dsh_reference_ic(const dsh_params* p, int species, const dsh_soa_view* out);
+
+/* ---- output (host only; replaces Data_out / Data_out_dust, kernel.cu:232-826) ---- */
+/* Writes <dir>/G%07d.dat and L_X%07d.dat (species 0) or D-k-%07d.dat and D-k-L_X%07d.dat
+   (dust species k) in the reference's text format for particles 0..pm of the view; t is
+   the time, tau and h the Init.txt values printed in the headers.  NULL fields print 0
+   (nn prints -1). */
+int dsh_write_data(const char* dir, int num, double t, double tau, double h, int species,
+                   const dsh_soa_view* view, int pm);
+/* Next-particle pointers of the reference's neighbour lists on its 2h grid
+   (kernel.cu:1767-1778, 1937-1995) for positions x, y, z of particles 0..pm. */
+int dsh_reference_nn(const dsh_params* p, const double* x, const double* y, const double* z, int pm, int* nn);
 /* sizeof of dsh_params, dsh_soa_view, dsh_tube_spec, dsh_server_opts (binding checks) */
 int dsh_struct_sizes(size_t out[4]);
 

@@ -89,8 +89,9 @@ struct Grid {
 
 int floordiv(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
 
-int axis_cell(double x, double o, double ic, int n) {
-    double t = (x - o) * ic;
+// int((x - o) / c): FP64 divide and truncation as kernel.cu:1949 and IDIC.h:28, clamped.
+int axis_cell(double x, double o, double c, int n) {
+    double t = (x - o) / c;
     if (!(t >= 0.0)) return 0;
     if (t >= (double)n) return n - 1;
     return (int)t;
@@ -167,9 +168,9 @@ Grid make_grid(const dsh_params& p, double hmin) {
 }
 
 void cell_of(const Grid& g, double x, double y, double z, int k[3]) {
-    k[0] = axis_cell(x, g.o[0], g.ic[0], g.n[0]);
-    k[1] = axis_cell(y, g.o[1], g.ic[1], g.n[1]);
-    k[2] = axis_cell(z, g.o[2], g.ic[2], g.n[2]);
+    k[0] = axis_cell(x, g.o[0], g.c[0], g.n[0]);
+    k[1] = axis_cell(y, g.o[1], g.c[1], g.n[1]);
+    k[2] = axis_cell(z, g.o[2], g.c[2], g.n[2]);
 }
 
 // kernel.cu:1937-1995: particles appended to their cell's list in increasing index,
@@ -456,8 +457,16 @@ void one_step(orc_sim& S) {
         for (int i = 0; i < g.n; ++i) g.csav[i] = std::sqrt(P.gamma * g.p[i] / g.rho[i]);
         for (int k = 0; k < nd; ++k) {
             Sp& d = S.sp[1 + k];
-            for (int i = 0; i < d.n; ++i)
-                if (d.ind[i] == 0) d.ts[i] = d.rho[i] / P.K[k];
+            if (P.drag_mode == 0) {
+                // t_stop_Kernel_IDIC (IDIC.h:78-91, launches kernel.cu:2049-2053): dust i uses
+                // the gas particle with the same index
+                for (int i = 0; i < d.n; ++i)
+                    if (d.ind[i] == 0)
+                        d.ts[i] = P.R_dust[k] / (std::sqrt(P.gamma * g.p[i] / g.rho[i]) * g.rho[i]);
+            } else {
+                for (int i = 0; i < d.n; ++i)
+                    if (d.ind[i] == 0) d.ts[i] = d.rho[i] / P.K[k];
+            }
         }
     }
     // averaging-cell sums (kernel.cu:2114-2190), all particles, index order
@@ -701,7 +710,7 @@ int orc_get_threads(void) {
 int orc_create(const dsh_params* p, orc_sim** out) {
     if (!p || !out) return DSH_ERR_ARG;
     if (p->n_dust < 1 || p->n_dust > DSH_MAX_DUST) return DSH_ERR_ARG;
-    if (p->drag_mode != 1) return DSH_ERR_UNSUPPORTED;
+    if (p->drag_mode != 0 && p->drag_mode != 1) return DSH_ERR_ARG;
     orc_sim* s = new orc_sim();
     s->P = *p;
     s->nsp = 1 + p->n_dust;
@@ -755,6 +764,9 @@ int orc_upload(orc_sim* S, int species, const dsh_soa_view* v, int pm) {
 
 int orc_step(orc_sim* S, int nsteps, double* t_out) {
     if (!S) return DSH_ERR_ARG;
+    if (S->P.drag_mode == 0)
+        for (int k = 1; k < S->nsp; ++k)
+            if (S->sp[k].n != S->sp[0].n) return DSH_ERR_ARG;   // dust i pairs with gas i
     if (!S->prepared) prepare(*S);
     for (int k = 0; k < nsteps && !S->done; ++k) one_step(*S);
     if (t_out) *t_out = S->t;

@@ -1,0 +1,148 @@
+"""The reference program's own run (init_Sod_X_Dust, two dust species, the stopping-time law
+t = R / (sqrt(gamma p / rho) rho) of the gas particle with the same index, global h, fixed
+tau) on the GPU against the FP64 oracle, and the Init.txt driver end to end.
+
+The particle spacing is coarsened (Particle_on_length 20 instead of 40) so the oracle finishes
+in seconds.  Multi-step runs also double h to 0.2, which keeps the reference's ratios (h = 4
+spacings, averaging cell 0.5 h = 2 spacings, frozen walls 2.5 h): with h = 0.1 at this spacing
+an averaging cell holds one particle per species, and the reference's test of the species-1
+count for species 2 (kernel.cu:2203) then sets eps_2 = 0 in cells that lost their species-2
+particle, which makes the implicit solve 0 * inf.  Both the oracle and the GPU reproduce that
+(third step); the GPU reports it as a non-finite tube."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from dustyshock3d_amd import Simulation, sim
+from dustyshock3d_amd.abi import ERR_NONFINITE, FP64, DshError
+from tests.parity_util import inject, momentum_error, rel_pointwise, take_state, vec_normwise
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+INIT = os.path.join(HERE, "golden", "Init.txt")
+SPECIES = (0, 1, 2)
+
+
+def _reference_case(ppl, h=0.1):
+    p = sim.read_init(INIT)
+    p.particles_per_length = ppl
+    p.h = h
+    p.precision = FP64
+    sim.reference_setup(p)
+    return p, {s: sim.reference_ic(p, s) for s in SPECIES}
+
+
+def test_reference_run_first_step_bit_exact_sets(oracle_mod):
+    p, state = _reference_case(20)
+    print(f"\nreference run, {len(state[0]['x'])} particles per species", flush=True)
+    o = inject(oracle_mod.OracleSim, p, state)
+    g = inject(Simulation, p, state)
+    g.set_stats(True)
+    o.step(1)
+    g.step(1)
+    for s in SPECIES:
+        ro, rg = o.download(s), g.download(s)
+        assert np.array_equal(rg["cell"], ro["cell"]), f"species {s}: cell keys differ"
+        assert np.array_equal(rg["nn"], ro["nn"]), f"species {s}: sorted order differs"
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co) and np.array_equal(hg, ho), f"species {s}: neighbour sets differ"
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"species {s}: rho {e_rho:.3e} v {e_v:.3e} neighbours {co[act].mean():.1f}")
+        assert e_rho <= 1e-12 and e_v <= 1e-10
+        if s > 0:
+            # stopping times of this step from the same-index gas particle (IDIC.h:78-91)
+            assert rel_pointwise(rg["t_stop"], ro["t_stop"]) <= 1e-12
+            assert np.array_equal(rg["t_stop"][~act], state[s]["t_stop"][~act])
+    e_p = momentum_error(g.conservation(), o.conservation(), take_state(o, SPECIES))
+    assert e_p <= 1e-9
+
+
+def test_reference_run_ten_steps(oracle_mod):
+    p, state = _reference_case(20, h=0.2)
+    o = inject(oracle_mod.OracleSim, p, state)
+    g = inject(Simulation, p, state)
+    o.step(10)
+    g.step(10)
+    so, sg = o.status(), g.status()
+    assert so["steps"] == sg["steps"] == 10 and sg["t"] == so["t"]
+    st = take_state(o, SPECIES)
+    for s in SPECIES:
+        ro, rg = st[s], g.download(s)
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"\nspecies {s} after 10 steps: rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= 1e-5 and e_v <= 1e-5
+        if s == 0:
+            assert rel_pointwise(rg["e"][act], ro["e"][act]) <= 1e-5
+    assert momentum_error(g.conservation(), o.conservation(), st) <= 1e-9
+
+
+def test_eps2_quirk_turns_non_finite_on_the_same_step(oracle_mod):
+    p, state = _reference_case(20)
+    o = inject(oracle_mod.OracleSim, p, state)
+    g = inject(Simulation, p, state)
+    for k in (1, 2):
+        o.step(1)
+        g.step(1)
+        assert np.all(np.isfinite(o.conservation())), k
+    o.step(1)
+    assert not np.all(np.isfinite(o.conservation()))
+    with pytest.raises(DshError) as e:
+        g.step(1)
+    assert e.value.code == ERR_NONFINITE
+
+
+def test_init_driver_writes_the_reference_files(tmp_path):
+    exe = os.path.join(ROOT, "dustyshock3d_amd", "dsh_run")
+    assert os.path.exists(exe), "dsh_run is built by __graft_entry__.build()"
+    lines = open(INIT).read().split("\n")
+    lines[9] = "20\t\t\t// Particle on 1 length"
+    lines[10] = "0.2\t\t\t//	h"
+    lines[13] = "0.01\t\t// Time end"
+    lines[14] = "0.005\t\t// Time interval to output in files"
+    init = tmp_path / "Init.txt"
+    init.write_text("\n".join(lines))
+    out = tmp_path / "Data"
+    out.mkdir()
+    r = subprocess.run([exe, str(init), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.split("\n")[:2] == ["Time 0.005 ", "Time 0.010 "]
+    names = sorted(os.listdir(out))
+    want = sorted(f"{stem}{n:07d}.dat" for n in range(3)
+                  for stem in ("G", "L_X", "D-1-", "D-1-L_X", "D-2-", "D-2-L_X"))
+    assert names == want
+    # the same run through the library API, written by the same writer
+    p = sim.read_init(str(init))
+    sim.reference_setup(p)
+    state = {s: sim.reference_ic(p, s) for s in SPECIES}
+    g = inject(Simulation, p, state)
+    g.step(1)
+    nn_ref = {}
+    for s in SPECIES:
+        d = g.download(s)
+        nn_ref[s] = sim.reference_nn(p, d["x"], d["y"], d["z"])
+    g.step(1)
+    mine = tmp_path / "Mine"
+    mine.mkdir()
+    for s in SPECIES:
+        d = g.download(s)
+        d["nn"] = nn_ref[s]
+        if s > 0:
+            d["ax"][:] = 0.0
+            d["ay"][:] = 0.0
+            d["az"][:] = 0.0
+        sim.write_data(str(mine), 2, g.status()["t"], p.tau, p.h, s, d)
+    for name in os.listdir(mine):
+        assert (mine / name).read_bytes() == (out / name).read_bytes(), name
+    # output 0 is the placement itself
+    g0 = np.loadtxt(out / "G0000000.dat", skiprows=3)
+    assert g0.shape == (len(state[0]["x"]), 14)
+    assert np.array_equal(g0[:, 12], state[0]["ind"]) and np.all(g0[:, 13] == -1)
