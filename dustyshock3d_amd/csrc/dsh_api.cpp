@@ -19,7 +19,7 @@ extern "C" {
 
 int dsh_create(const dsh_params* params, dsh_ctx** out) {
     if (!params || !out) return DSH_ERR_ARG;
-    if (params->n_dust < 1 || params->n_dust > DSH_MAX_DUST) return DSH_ERR_ARG;
+    if (params->n_dust < 0 || params->n_dust > DSH_MAX_DUST) return DSH_ERR_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return DSH_ERR_HIP;
     dsh_ctx* c = new dsh_ctx();

@@ -162,7 +162,7 @@ int Engine::prepare() {
     for (int t = 0; t < T; ++t) {
         const dsh_params& P = params_[t];
         if (P.precision != precision_) return fail(DSH_ERR_ARG, "tubes of one batch must share the precision");
-        if (P.n_dust < 1 || P.n_dust > kMaxDust) return fail(DSH_ERR_ARG, "n_dust must be 1 or 2");
+        if (P.n_dust < 0 || P.n_dust > kMaxDust) return fail(DSH_ERR_ARG, "n_dust must be 0, 1 or 2");
         if (P.drag_mode != 0 && P.drag_mode != 1) return fail(DSH_ERR_ARG, "drag_mode must be 0 or 1");
         for (int s = 0; s <= P.n_dust; ++s)
             if (!staged_[(size_t)t * kMaxSpecies + s].set) return fail(DSH_ERR_STATE, "species not uploaded");
@@ -526,10 +526,24 @@ int Engine::sync() {
     return refresh_dyn();
 }
 
+// Steps are enqueued in chunks; between chunks the host reads the tube states and stops
+// once every tube has finished, so a caller may ask for more steps than a run needs.
 int Engine::run(int nsteps) {
+    constexpr int kChunk = 64;
     int rc;
-    if ((rc = launch_steps(nsteps))) return rc;
-    if ((rc = sync())) return rc;
+    for (int left = nsteps; left > 0;) {
+        const int k = left < kChunk ? left : kChunk;
+        if ((rc = launch_steps(k))) return rc;
+        if ((rc = sync())) return rc;
+        left -= k;
+        bool all_done = true;
+        for (const TubeDyn& d : dyn_h_) all_done = all_done && (d.done || d.err);
+        if (all_done) break;
+    }
+    if (nsteps == 0) {
+        if ((rc = launch_steps(0))) return rc;
+        if ((rc = sync())) return rc;
+    }
     return error_status();
 }
 

@@ -134,6 +134,20 @@ template <typename R> __device__ __forceinline__ R spline_dw(R q) {
     return q <= R(1) ? inner : (q <= R(2) ? outer : R(0));
 }
 
+// dW of a signed coordinate difference, without the 1/(pi h^4) factor (kernel.cu:24-38: odd in u).
+template <typename R> __device__ __forceinline__ R spline_dw_signed(R u) {
+    return u < R(0) ? -spline_dw<R>(-u) : spline_dw<R>(u);
+}
+
+// de/dt weight of the plain ForceKernel (kernel.cu:182-187): each velocity component times
+// dW of the same coordinate difference x_i - x_j = -d (gas-only runs).
+template <typename R>
+__device__ __forceinline__ R plain_edot_weight(R dvx, R dvy, R dvz, R fx, R fy, R fz, R ih) {
+    const R c4 = (R)kInvPi * (ih * ih * ih * ih);
+    return (dvx * (c4 * spline_dw_signed<R>(-fx * ih)) + dvy * (c4 * spline_dw_signed<R>(-fy * ih))) +
+           dvz * (c4 * spline_dw_signed<R>(-fz * ih));
+}
+
 // ---------------------------------------------------------------- neighbour traversal
 // Visits every candidate j within ceil(reach / c_a) cells of (kx,ky,kz) along each axis a,
 // periodic in y/z (each offset is a distinct (cell, image) pair, so thin tubes get every
@@ -631,6 +645,7 @@ struct ForcePair {
     const double* __restrict__ CS;
     double hi, vxi, vyi, vzi;
     R rhoi, pr2i, csi, mi, alpha, beta, eps;
+    bool plain;   // gas-only run: de/dt of the plain ForceKernel
     Acc<R> ax, ay, az, ed;
     __device__ __forceinline__ void pair(int j, double dx, double dy, double dz, double d2, int, int) {
         if (!(d2 > 0.0)) return;
@@ -649,7 +664,12 @@ struct ForcePair {
         const R id4 = ih * ih * ih * ih;
         const R dw = (R)kInvPi * id4 * spline_dw<R>(dist * ih);
         const R dwr = dw * rcp<R>(dist);
-        ed.add((mi * pr2i + mi * fnu * R(0.5)) * nu * dwr);
+        if (plain) {
+            const R w = plain_edot_weight<R>((R)(vxi - VX[j]), (R)(vyi - VY[j]), (R)(vzi - VZ[j]), fx, fy, fz, ih);
+            ed.add((mi * pr2i + mi * fnu * R(0.5)) * w);
+        } else {
+            ed.add((mi * pr2i + mi * fnu * R(0.5)) * nu * dwr);
+        }
         const R coef = (R)M[j] * (pr2i + (R)PR2[j] + fnu) * dwr;
         ax.add(fx * coef);
         ay.add(fy * coef);
@@ -690,7 +710,7 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
     ForcePair<R> f{B.m + wo, H, B.vx + wo, B.vy + wo, B.vz + wo, B.rho + wo, pr2, csav,
                    H[q], B.vx[wo + q], B.vy[wo + q], B.vz[wo + q],
                    (R)B.rho[wo + q], (R)pr2[q], (R)csav[q], (R)B.m[wo + q],
-                   (R)T.alpha, (R)T.beta, (R)T.eps};
+                   (R)T.alpha, (R)T.beta, (R)T.eps, T.nsp == 1};
     for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
     B.ax[wo + q] = f.ax.get();
     B.ay[wo + q] = f.ay.get();
@@ -771,7 +791,7 @@ struct TileLds {
 template <typename T>
 __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
                                             T pr2i, T pr2j, T csi, T csj, T mi, T mj, T alpha, T beta, T epsv,
-                                            T& coef, T& edc) {
+                                            bool plain, T& coef, T& edc) {
     const T ih = rcp<T>(hh);
     // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
     const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
@@ -783,7 +803,10 @@ __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dv
     }
     const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw<T>(dist * ih);
     const T dwr = dw * rcp<T>(dist);
-    edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
+    if (plain)
+        edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
+    else
+        edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
     coef = mj * (pr2i + pr2j + fnu) * dwr;
 }
 
@@ -844,6 +867,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
     }
     const R rhoi = (R)rhoi_d, pr2i = (R)pr2i_d, csi = (R)csi_d, mi = (R)mi_d;
     const R alpha = (R)T.alpha, beta = (R)T.beta, epsv = (R)T.eps;
+    const bool plain = T.nsp == 1;   // gas-only run: de/dt of the plain ForceKernel
 
     bool pending = active;
     while (true) {
@@ -975,8 +999,8 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                     double coef, edc;
                                     force_terms<double>(sqrt(dd), hhd, dx, dy, dz, vxi - L.vx[u], vyi - L.vy[u],
                                                         vzi - L.vz[u], rhoi_d, B.rho[g], pr2i_d, pr2[pb + u], csi_d,
-                                                        csav[pb + u], mi_d, B.m[g], T.alpha, T.beta, T.eps, coef,
-                                                        edc);
+                                                        csav[pb + u], mi_d, B.m[g], T.alpha, T.beta, T.eps, plain,
+                                                        coef, edc);
                                     d0 += dx * coef;
                                     d1 += dy * coef;
                                     d2s += dz * coef;
@@ -986,7 +1010,8 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                     const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
                                     force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, (R)(vxi - L.vx[u]),
                                                    (R)(vyi - L.vy[u]), (R)(vzi - L.vz[u]), rhoi, L.rho[u], pr2i,
-                                                   L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha, beta, epsv, coef, edc);
+                                                   L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha, beta, epsv, plain, coef,
+                                                   edc);
                                     a0.add(fx * coef);
                                     a1.add(fy * coef);
                                     a2.add(fz * coef);
@@ -1092,7 +1117,8 @@ __device__ __forceinline__ void idic_solve(int nd, double vg, const double* vd, 
 
 // One wave per averaging cell: sums over its f^3 neighbour cells for every species,
 // implicit drag solve, then the per-particle kick (FP64), drift, energy and h update,
-// and the cell's conservation / CFL partials.
+// and the cell's conservation / CFL partials.  A gas-only tube takes the explicit kick
+// v += A tau of the plain scheme instead of the drag solve.
 __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restrict__ tc,
                                               const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                               const int* __restrict__ start, const int* __restrict__ ginv,
@@ -1171,16 +1197,19 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
         const int test = (k == 1 && T.eps2_quirk) ? ndc[0] : ndc[k];
         ep[k] = (ng > 0 && test > 0) ? ndc[k] * T.mu[k] / ng : T.eps_fb[k];
     }
-    double vgh[3], vdh[kMaxDust][3];
-    for (int a = 0; a < 3; ++a) {
-        double vdc[kMaxDust] = {vdm[0][a], nd > 1 ? vdm[1][a] : 0.0};
-        double out[kMaxDust] = {0.0, 0.0};
-        idic_solve(nd, vg[a], vdc, psi[a], tb, ep, tau, &vgh[a], out);
-        for (int k = 0; k < nd; ++k) vdh[k][a] = out[k];
-    }
+    double vgh[3] = {0.0, 0.0, 0.0}, vdh[kMaxDust][3];
+    if (nd > 0)
+        for (int a = 0; a < 3; ++a) {
+            double vdc[kMaxDust] = {vdm[0][a], nd > 1 ? vdm[1][a] : 0.0};
+            double out[kMaxDust] = {0.0, 0.0};
+            idic_solve(nd, vg[a], vdc, psi[a], tb, ep, tau, &vgh[a], out);
+            for (int k = 0; k < nd; ++k) vdh[k][a] = out[k];
+        }
     // gas velocity terms (IDIC.h:301-309)
     double tmp1 = 0.0, tmpv[3] = {0.0, 0.0, 0.0};
-    if (nd == 1) {
+    if (nd == 0) {
+        // gas only: the explicit kick of MoveKernel below
+    } else if (nd == 1) {
         tmp1 = tmp1 + ep[0] / tb[0];
         for (int a = 0; a < 3; ++a) tmpv[a] = tmpv[a] + ep[0] * vdh[0][a] / tb[0];
     } else {
@@ -1200,7 +1229,12 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
                 const double m = B.m[g];
                 double e = 0.0;
                 if (B.ind[g] == 0) {
-                    if (s == 0) {
+                    if (s == 0 && nd == 0) {
+                        // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
+                        vx = vx + B.ax[g] * tau;
+                        vy = vy + B.ay[g] * tau;
+                        vz = vz + B.az[g] * tau;
+                    } else if (s == 0) {
                         vx = (vx / tau + tmpv[0] + psi[0]) / (1.0 / tau + tmp1);
                         vy = (vy / tau + tmpv[1] + psi[1]) / (1.0 / tau + tmp1);
                         vz = (vz / tau + tmpv[2] + psi[2]) / (1.0 / tau + tmp1);

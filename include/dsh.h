@@ -74,7 +74,8 @@ typedef struct dsh_params {
     double cs;                    /* Speed_of_sound_gas (kernel.cu:1744) */
     int    h_mode;                /* 0: global h, 1: h_i = eta (m_i / rho_i)^(1/3) */
     double eta;
-    int    n_dust;                /* 1 or 2 dust species */
+    int    n_dust;                /* dust species: 1 or 2; 0 runs gas alone with the plain explicit
+                                     scheme of kernel.cu:48-228 */
     int    drag_mode;             /* 1: t_stop = rho_dust / K (only mode implemented) */
     double K[DSH_MAX_DUST];
     double R_dust[DSH_MAX_DUST];  /* reserved for the reference stopping-time law (IDIC.h:78-91) */

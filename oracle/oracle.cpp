@@ -274,10 +274,12 @@ void density(orc_sim& S, int sidx) {
     }
 }
 
-// IDIC.h:198-260
+// IDIC.h:198-260; gas-only runs take de/dt of the plain ForceKernel (kernel.cu:182-187),
+// which weights each velocity component with dW of that coordinate difference.
 void force(orc_sim& S) {
     Sp& s = S.sp[0];
     const double alpha = S.P.alpha, beta = S.P.beta, eps = S.P.eps;
+    const bool plain = S.nsp == 1;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < s.n; ++i) {
         if (s.ind[i] != 0) {
@@ -288,6 +290,7 @@ void force(orc_sim& S) {
             const Sp& s;
             int i;
             double alpha, beta, eps;
+            bool plain;
             double ax = 0.0, ay = 0.0, az = 0.0, ed = 0.0;
             void begin_cell() {}
             void end_cell() {}
@@ -306,8 +309,13 @@ void force(orc_sim& S) {
                     F_nu = (-alpha * Cnu * nu + beta * nu * nu) / (0.5 * (s.rho[i] + s.rho[j]));
                 }
                 const double dw = dW(dist, hh);
-                ed = ed + (s.m[i] * s.p[i] / (s.rho[i] * s.rho[i]) + s.m[i] * F_nu / 2.0) *
-                              (nu1x + nu1y + nu1z) * dw / dist;
+                if (plain)
+                    ed = ed + (s.m[i] * s.p[i] / (s.rho[i] * s.rho[i]) + s.m[i] * F_nu / 2.0) *
+                                  ((s.vx[i] - s.vx[j]) * dW(-dx, hh) + (s.vy[i] - s.vy[j]) * dW(-dy, hh) +
+                                   (s.vz[i] - s.vz[j]) * dW(-dz, hh));
+                else
+                    ed = ed + (s.m[i] * s.p[i] / (s.rho[i] * s.rho[i]) + s.m[i] * F_nu / 2.0) *
+                                  (nu1x + nu1y + nu1z) * dw / dist;
                 const double A = s.m[j] *
                                  (s.p[i] / (s.rho[i] * s.rho[i]) + s.p[j] / (s.rho[j] * s.rho[j]) + F_nu) *
                                  dw;
@@ -315,7 +323,7 @@ void force(orc_sim& S) {
                 ay = ay + dy * A / dist;
                 az = az + dz * A / dist;
             }
-        } acc{s, i, alpha, beta, eps};
+        } acc{s, i, alpha, beta, eps, plain};
         for_neighbours(s, S.g, i, acc);
         s.ax[i] = acc.ax;
         s.ay[i] = acc.ay;
@@ -524,66 +532,79 @@ void one_step(orc_sim& S) {
         }
     // force (kernel.cu:2233-2266)
     force(S);
-    // Psi = cell mean of gas acceleration (kernel.cu:2294-2309)
-    {
-        Sp& g = S.sp[0];
-        for (int i = 0; i < g.n; ++i) {
-            int c = cg[i];
-            ps[3 * c + 0] += g.ax[i];
-            ps[3 * c + 1] += g.ay[i];
-            ps[3 * c + 2] += g.az[i];
-        }
-        for (int c = 0; c < A; ++c)
-            if (ng[c] > 0)
-                for (int a = 0; a < 3; ++a) ps[3 * c + a] = ps[3 * c + a] / ng[c];
-    }
-    // implicit solve (kernel.cu:2310-2437)
-    std::vector<double> vgh(3 * (size_t)A), vdh(3 * (size_t)A * nd);
-    for (int c = 0; c < A; ++c) {
-        for (int a = 0; a < 3; ++a) {
-            double vdc[DSH_MAX_DUST], tbc[DSH_MAX_DUST], epc[DSH_MAX_DUST], out[DSH_MAX_DUST];
-            for (int k = 0; k < nd; ++k) {
-                size_t ck = (size_t)c * nd + k;
-                vdc[k] = vd[3 * ck + a];
-                tbc[k] = tb[ck];
-                epc[k] = ep[ck];
-            }
-            double g;
-            idic_solve(nd, vg[3 * c + a], vdc, ps[3 * c + a], tbc, epc, tau, &g, out);
-            vgh[3 * c + a] = g;
-            for (int k = 0; k < nd; ++k) vdh[3 * ((size_t)c * nd + k) + a] = out[k];
-        }
-    }
-    // dust velocities (IDIC.h:264-284)
-    for (int k = 0; k < nd; ++k) {
-        Sp& d = S.sp[1 + k];
-        for (int i = 0; i < d.n; ++i) {
-            if (d.ind[i] != 0) continue;
-            int c = cd[k][i];
-            double t = tb[(size_t)c * nd + k];
-            double* v[3] = {&d.vx[i], &d.vy[i], &d.vz[i]};
-            for (int a = 0; a < 3; ++a) *v[a] = (*v[a] / tau + vgh[3 * c + a] / t) / (1.0 / tau + 1.0 / t);
-        }
-    }
-    // gas velocities (IDIC.h:287-315)
-    {
+    std::vector<double> vgh, vdh;
+    if (nd == 0) {
+        // gas only: the plain explicit kick of MoveKernel (kernel.cu:205-228)
         Sp& g = S.sp[0];
         for (int i = 0; i < g.n; ++i) {
             if (g.ind[i] != 0) continue;
-            int c = cg[i];
-            double tmp1 = 0.0;
-            double tmp[3] = {0.0, 0.0, 0.0};
-            if (nd == 1) {
-                tmp1 = tmp1 + ep[c] / tb[c];
-                for (int a = 0; a < 3; ++a) tmp[a] = tmp[a] + ep[c] * vdh[3 * (size_t)c + a] / tb[c];
-            } else {
-                size_t c0 = (size_t)c * 2, c1 = c0 + 1;
-                tmp1 = tmp1 + ep[c0] / tb[c0] + ep[c1] / tb[c1];
-                for (int a = 0; a < 3; ++a)
-                    tmp[a] = tmp[a] + ep[c0] * vdh[3 * c0 + a] / tb[c0] + ep[c1] * vdh[3 * c1 + a] / tb[c1];
+            g.vx[i] = g.vx[i] + g.ax[i] * tau;
+            g.vy[i] = g.vy[i] + g.ay[i] * tau;
+            g.vz[i] = g.vz[i] + g.az[i] * tau;
+        }
+    } else {
+        // Psi = cell mean of gas acceleration (kernel.cu:2294-2309)
+        {
+            Sp& g = S.sp[0];
+            for (int i = 0; i < g.n; ++i) {
+                int c = cg[i];
+                ps[3 * c + 0] += g.ax[i];
+                ps[3 * c + 1] += g.ay[i];
+                ps[3 * c + 2] += g.az[i];
             }
-            double* v[3] = {&g.vx[i], &g.vy[i], &g.vz[i]};
-            for (int a = 0; a < 3; ++a) *v[a] = (*v[a] / tau + tmp[a] + ps[3 * c + a]) / (1.0 / tau + tmp1);
+            for (int c = 0; c < A; ++c)
+                if (ng[c] > 0)
+                    for (int a = 0; a < 3; ++a) ps[3 * c + a] = ps[3 * c + a] / ng[c];
+        }
+        // implicit solve (kernel.cu:2310-2437)
+        vgh.assign(3 * (size_t)A, 0.0);
+        vdh.assign(3 * (size_t)A * nd, 0.0);
+        for (int c = 0; c < A; ++c) {
+            for (int a = 0; a < 3; ++a) {
+                double vdc[DSH_MAX_DUST], tbc[DSH_MAX_DUST], epc[DSH_MAX_DUST], out[DSH_MAX_DUST];
+                for (int k = 0; k < nd; ++k) {
+                    size_t ck = (size_t)c * nd + k;
+                    vdc[k] = vd[3 * ck + a];
+                    tbc[k] = tb[ck];
+                    epc[k] = ep[ck];
+                }
+                double g;
+                idic_solve(nd, vg[3 * c + a], vdc, ps[3 * c + a], tbc, epc, tau, &g, out);
+                vgh[3 * c + a] = g;
+                for (int k = 0; k < nd; ++k) vdh[3 * ((size_t)c * nd + k) + a] = out[k];
+            }
+        }
+        // dust velocities (IDIC.h:264-284)
+        for (int k = 0; k < nd; ++k) {
+            Sp& d = S.sp[1 + k];
+            for (int i = 0; i < d.n; ++i) {
+                if (d.ind[i] != 0) continue;
+                int c = cd[k][i];
+                double t = tb[(size_t)c * nd + k];
+                double* v[3] = {&d.vx[i], &d.vy[i], &d.vz[i]};
+                for (int a = 0; a < 3; ++a) *v[a] = (*v[a] / tau + vgh[3 * c + a] / t) / (1.0 / tau + 1.0 / t);
+            }
+        }
+        // gas velocities (IDIC.h:287-315)
+        {
+            Sp& g = S.sp[0];
+            for (int i = 0; i < g.n; ++i) {
+                if (g.ind[i] != 0) continue;
+                int c = cg[i];
+                double tmp1 = 0.0;
+                double tmp[3] = {0.0, 0.0, 0.0};
+                if (nd == 1) {
+                    tmp1 = tmp1 + ep[c] / tb[c];
+                    for (int a = 0; a < 3; ++a) tmp[a] = tmp[a] + ep[c] * vdh[3 * (size_t)c + a] / tb[c];
+                } else {
+                    size_t c0 = (size_t)c * 2, c1 = c0 + 1;
+                    tmp1 = tmp1 + ep[c0] / tb[c0] + ep[c1] / tb[c1];
+                    for (int a = 0; a < 3; ++a)
+                        tmp[a] = tmp[a] + ep[c0] * vdh[3 * c0 + a] / tb[c0] + ep[c1] * vdh[3 * c1 + a] / tb[c1];
+                }
+                double* v[3] = {&g.vx[i], &g.vy[i], &g.vz[i]};
+                for (int a = 0; a < 3; ++a) *v[a] = (*v[a] / tau + tmp[a] + ps[3 * c + a]) / (1.0 / tau + tmp1);
+            }
         }
     }
     // drift (IDIC.h:319-333), periodic wrap, energy (IDIC.h:335-346), smoothing length
@@ -709,7 +730,7 @@ int orc_get_threads(void) {
 
 int orc_create(const dsh_params* p, orc_sim** out) {
     if (!p || !out) return DSH_ERR_ARG;
-    if (p->n_dust < 1 || p->n_dust > DSH_MAX_DUST) return DSH_ERR_ARG;
+    if (p->n_dust < 0 || p->n_dust > DSH_MAX_DUST) return DSH_ERR_ARG;
     if (p->drag_mode != 0 && p->drag_mode != 1) return DSH_ERR_ARG;
     orc_sim* s = new orc_sim();
     s->P = *p;
