@@ -18,32 +18,14 @@ struct dsh_batch {
 
 namespace dsh {
 
-// Host set-up of one tube into an engine slot (the arrays live only for the call).
+// Seeded set-up of one tube into an engine slot: each species is generated on the device
+// at prepare() (no host arrays, no upload).
 int upload_tube_spec(Engine& eng, int tube, const dsh_tube_spec& spec) {
-    int n[3];
-    int rc = dsh_tube_count(&spec, n);
-    if (rc) return rc;
     for (int s = 0; s < 2; ++s) {
-        const size_t m = (size_t)n[s];
-        std::vector<double> f[13];
-        for (auto& v : f) v.resize(m);
-        std::vector<int> ind(m);
-        dsh_soa_view v{};
-        v.x = f[0].data();
-        v.y = f[1].data();
-        v.z = f[2].data();
-        v.vx = f[3].data();
-        v.vy = f[4].data();
-        v.vz = f[5].data();
-        v.m = f[6].data();
-        v.rho = f[7].data();
-        v.p = f[8].data();
-        v.e = f[9].data();
-        v.h = f[10].data();
-        v.t_stop = f[11].data();
-        v.ind = ind.data();
-        if ((rc = dsh_tube_ic(&spec, s, &v))) return rc;
-        if ((rc = eng.upload(tube, s, &v, n[s] - 1))) return rc;
+        TubeIC c;
+        int rc = make_tube_ic(spec, s, c);
+        if (rc) return rc;
+        if ((rc = eng.stage_ic(tube, s, c))) return rc;
     }
     return DSH_OK;
 }

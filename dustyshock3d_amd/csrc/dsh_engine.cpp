@@ -154,6 +154,20 @@ int Engine::upload(int tube, int species, const dsh_soa_view* v, int pm) {
     return DSH_OK;
 }
 
+int Engine::stage_ic(int tube, int species, const TubeIC& ic) {
+    if (prepared_) return fail(DSH_ERR_STATE, "set-up after the first step");
+    if (tube < 0 || tube >= ntubes()) return fail(DSH_ERR_ARG, "tube index out of range");
+    if (species < 0 || species > params_[tube].n_dust) return fail(DSH_ERR_ARG, "species index out of range");
+    if (ic.n < 0 || ic.n0 < 0 || ic.n0 > ic.n) return fail(DSH_ERR_ARG, "bad set-up counts");
+    HostSpecies& q = staged_[(size_t)tube * kMaxSpecies + species];
+    q = HostSpecies{};
+    q.n = ic.n;
+    q.set = true;
+    q.device_ic = true;
+    q.ic = ic;
+    return DSH_OK;
+}
+
 int Engine::prepare() {
     if (prepared_) return DSH_OK;
     const int T = ntubes();
@@ -181,7 +195,12 @@ int Engine::prepare() {
             double hmin = 1e300;
             for (int s = 0; s <= P.n_dust; ++s) {
                 const HostSpecies& q = staged_[(size_t)t * kMaxSpecies + s];
-                for (int i = 0; i < q.n; ++i) hmin = std::min(hmin, q.h[i]);
+                if (q.device_ic) {
+                    if (q.ic.n0 > 0) hmin = std::min(hmin, q.ic.hloc[0]);
+                    if (q.ic.n > q.ic.n0) hmin = std::min(hmin, q.ic.hloc[1]);
+                } else {
+                    for (int i = 0; i < q.n; ++i) hmin = std::min(hmin, q.h[i]);
+                }
             }
             build_grid(P, hmin, C);
         }
@@ -364,6 +383,10 @@ int Engine::prepare() {
             const size_t off = (size_t)tc_h_[t].p_off[s];
             const size_t n = (size_t)q.n;
             if (n == 0) continue;
+            if (q.device_ic) {
+                if ((rc = hip_check(launch_tube_ic(q.ic, B, (long long)off, stream_), "k_tube_ic"))) return rc;
+                continue;
+            }
             struct {
                 double* dst;
                 const std::vector<double>& src;

@@ -15,6 +15,8 @@ namespace dsh {
 struct HostSpecies {
     int n = 0;
     bool set = false;
+    bool device_ic = false;   // generated on the device at prepare() from ic
+    TubeIC ic{};
     std::vector<double> x, y, z, vx, vy, vz, m, h, e, rho, p, ts;
     std::vector<int> ind;
 };
@@ -30,6 +32,8 @@ public:
     Engine& operator=(const Engine&) = delete;
 
     int upload(int tube, int species, const dsh_soa_view* v, int pm);
+    // Stage a seeded shock-tube species to be generated on the device at prepare().
+    int stage_ic(int tube, int species, const TubeIC& ic);
     // Take tube src_tube of a prepared engine, state and clock, without a host round
     // trip: at prepare() its arrays are copied device to device and its grid is kept.
     int adopt(int tube, const Engine& src, int src_tube);

@@ -7,31 +7,9 @@
 #include <cstring>
 
 #include "../../include/dsh.h"
+#include "dsh_ic.h"
 
 namespace {
-
-uint64_t mix64(uint64_t x) {
-    x += 0x9E3779B97F4A7C15ull;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    return x ^ (x >> 31);
-}
-
-// Counter-based stream keyed by (seed, tube, species, id, axis): the set-up of one
-// particle does not depend on how many others were generated before it.
-double jitter_unit(uint64_t seed, int tube, int species, int id, int axis) {
-    uint64_t z = mix64(seed ^ ((uint64_t)(uint32_t)tube * 0xD1B54A32D192ED03ull));
-    z = mix64(z ^ ((uint64_t)(uint32_t)(species * 4 + axis) * 0x8CB92BA72F3D8DD7ull));
-    z = mix64(z ^ ((uint64_t)(uint32_t)id * 0xA0761D6478BD642Full));
-    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
-    return 2.0 * u - 1.0;
-}
-
-double wrap_periodic(double y, double w) {
-    if (y < 0.0) y = y + w;
-    if (y >= w) y = y - w;
-    return y;
-}
 
 // Lattice of one tube: x <= 0 at spacing dx (rows anchored at x = 0 going -x, as
 // kernel.cu:1421), x > 0 at 2 dx starting at +2dx (kernel.cu:1461); the cross-section
@@ -69,6 +47,47 @@ bool spec_ok(const dsh_tube_spec* t) {
 }
 
 }  // namespace
+
+namespace dsh {
+
+int make_tube_ic(const dsh_tube_spec& t, int species, TubeIC& c) {
+    if (species < 0 || species > 1) return DSH_ERR_ARG;
+    const Lattice L = lattice(t);
+    const long long n0 = (long long)L.nx[0] * L.ny[0] * L.nz[0];
+    const long long n1 = (long long)L.nx[1] * L.ny[1] * L.nz[1];
+    if (n0 + n1 > 2000000000ll) return DSH_ERR_ARG;
+    const double ratio = species == 0 ? 1.0 : t.dust_to_gas;
+    const double cs = 1.0, gam = 1.4;
+    c = TubeIC{};
+    c.n = (int)(n0 + n1);
+    c.n0 = (int)n0;
+    for (int s = 0; s < 2; ++s) {
+        c.nx[s] = L.nx[s];
+        c.ny[s] = L.ny[s];
+        c.nz[s] = L.nz[s];
+        c.dx[s] = L.dx[s];
+        c.dy[s] = L.dy[s];
+        c.dz[s] = L.dz[s];
+        c.hloc[s] = 1.2 * std::cbrt(L.dx[s] * L.dy[s] * L.dz[s]);
+        c.rho[s] = L.rho[s] * ratio;
+        c.p[s] = species == 0 ? cs * cs * c.rho[s] : 0.0;
+    }
+    c.m = L.m * ratio;
+    c.e = species == 0 ? cs * cs / (gam - 1.0) : 0.0;
+    c.ts_num_k = t.K;
+    c.x_min = t.x_min;
+    c.x_max = t.x_max;
+    c.y_width = t.y_width;
+    c.z_width = t.z_width;
+    c.jitter = t.jitter;
+    c.frozen_h = t.frozen_h;
+    c.seed = t.seed;
+    c.tube_index = t.tube_index;
+    c.species = species;
+    return DSH_OK;
+}
+
+}  // namespace dsh
 
 // ---- the reference's own problem: init_Sod_X_Dust (kernel.cu:1391-1575) ----
 //
@@ -234,46 +253,30 @@ int dsh_tube_count(const dsh_tube_spec* t, int* n) {
 
 int dsh_tube_ic(const dsh_tube_spec* t, int species, const dsh_soa_view* v) {
     if (!spec_ok(t) || !v || species < 0 || species > 1) return DSH_ERR_ARG;
-    const Lattice L = lattice(*t);
-    const double ratio = species == 0 ? 1.0 : t->dust_to_gas;
-    const double cs = 1.0, gam = 1.4;
-    int id = 0;
-    for (int side = 0; side < 2; ++side) {
-        const double dx = L.dx[side], dy = L.dy[side], dz = L.dz[side];
-        const double hloc = 1.2 * std::cbrt(dx * dy * dz);
-        const double rho = L.rho[side] * ratio;
-        for (int i = 0; i < L.nx[side]; ++i)
-            for (int j = 0; j < L.ny[side]; ++j)
-                for (int k = 0; k < L.nz[side]; ++k, ++id) {
-                    double x = side == 0 ? 0.0 - (double)i * dx : dx + (double)i * dx;
-                    double y = 0.0 + (double)j * dy;
-                    double z = 0.0 + (double)k * dz;
-                    x = x + t->jitter * dx * jitter_unit(t->seed, t->tube_index, species, id, 0);
-                    y = y + t->jitter * dy * jitter_unit(t->seed, t->tube_index, species, id, 1);
-                    z = z + t->jitter * dz * jitter_unit(t->seed, t->tube_index, species, id, 2);
-                    y = wrap_periodic(y, t->y_width);
-                    z = wrap_periodic(z, t->z_width);
-                    if (v->x) v->x[id] = x;
-                    if (v->y) v->y[id] = y;
-                    if (v->z) v->z[id] = z;
-                    if (v->vx) v->vx[id] = 0.0;
-                    if (v->vy) v->vy[id] = 0.0;
-                    if (v->vz) v->vz[id] = 0.0;
-                    if (v->ax) v->ax[id] = 0.0;
-                    if (v->ay) v->ay[id] = 0.0;
-                    if (v->az) v->az[id] = 0.0;
-                    if (v->m) v->m[id] = L.m * ratio;
-                    if (v->rho) v->rho[id] = rho;
-                    if (v->p) v->p[id] = species == 0 ? cs * cs * rho : 0.0;
-                    if (v->e) v->e[id] = species == 0 ? cs * cs / (gam - 1.0) : 0.0;
-                    if (v->h) v->h[id] = hloc;
-                    if (v->t_stop) v->t_stop[id] = species == 0 ? 0.0 : rho / t->K;
-                    if (v->edot) v->edot[id] = 0.0;
-                    if (v->ind)
-                        v->ind[id] = (x < t->x_min + t->frozen_h * hloc || x > t->x_max - t->frozen_h * hloc) ? 1 : 0;
-                    if (v->nn) v->nn[id] = -1;
-                    if (v->cell) v->cell[id] = -1;
-                }
+    dsh::TubeIC c;
+    const int rc = dsh::make_tube_ic(*t, species, c);
+    if (rc) return rc;
+    for (int id = 0; id < c.n; ++id) {
+        const dsh::ICParticle q = dsh::tube_ic_particle(c, id);
+        if (v->x) v->x[id] = q.x;
+        if (v->y) v->y[id] = q.y;
+        if (v->z) v->z[id] = q.z;
+        if (v->vx) v->vx[id] = 0.0;
+        if (v->vy) v->vy[id] = 0.0;
+        if (v->vz) v->vz[id] = 0.0;
+        if (v->ax) v->ax[id] = 0.0;
+        if (v->ay) v->ay[id] = 0.0;
+        if (v->az) v->az[id] = 0.0;
+        if (v->m) v->m[id] = q.m;
+        if (v->rho) v->rho[id] = q.rho;
+        if (v->p) v->p[id] = q.p;
+        if (v->e) v->e[id] = q.e;
+        if (v->h) v->h[id] = q.h;
+        if (v->t_stop) v->t_stop[id] = q.ts;
+        if (v->edot) v->edot[id] = 0.0;
+        if (v->ind) v->ind[id] = q.ind;
+        if (v->nn) v->nn[id] = -1;
+        if (v->cell) v->cell[id] = -1;
     }
     return DSH_OK;
 }

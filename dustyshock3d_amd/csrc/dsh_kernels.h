@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "dsh_ic.h"
 #include "dsh_types.h"
 
 namespace dsh {
@@ -62,6 +63,8 @@ hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s);
 hipError_t launch_idic(const StepArgs& a, hipStream_t s);
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s);
 hipError_t launch_flip(const StepArgs& a, hipStream_t s);
+// seeded shock-tube set-up of one (tube, species) straight into buffer slots off..off+n-1
+hipError_t launch_tube_ic(const TubeIC& c, const PBuf& B, long long off, hipStream_t s);
 
 int scan_blocks(int n);
 size_t mask_words(int gas_chunks);

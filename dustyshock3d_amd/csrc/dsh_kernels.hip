@@ -1373,6 +1373,29 @@ __global__ __launch_bounds__(kBS) void k_finalize(const TubeConst* __restrict__ 
 
 __global__ void k_flip(int* cur) { *cur = 1 - *cur; }
 
+// The seeded lattice set-up of one (tube, species), one thread per particle, written in
+// original order with id = index (what an upload of dsh_tube_ic's arrays would store).
+__global__ __launch_bounds__(kBS) void k_tube_ic(TubeIC c, PBuf B, long long off) {
+    const int id = blockIdx.x * kBS + threadIdx.x;
+    if (id >= c.n) return;
+    const ICParticle q = tube_ic_particle(c, id);
+    const long long g = off + id;
+    B.x[g] = q.x;
+    B.y[g] = q.y;
+    B.z[g] = q.z;
+    B.vx[g] = 0.0;
+    B.vy[g] = 0.0;
+    B.vz[g] = 0.0;
+    B.m[g] = q.m;
+    B.h[g] = q.h;
+    B.e[g] = q.e;
+    B.rho[g] = q.rho;
+    B.p[g] = q.p;
+    B.ts[g] = q.ts;
+    B.ind[g] = q.ind;
+    B.id[g] = id;
+}
+
 inline dim3 grid1(int n) { return dim3((unsigned)((n + kBS - 1) / kBS)); }
 
 }  // namespace
@@ -1492,6 +1515,12 @@ hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
 
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.cur, a.part);
+    return hipGetLastError();
+}
+
+hipError_t launch_tube_ic(const TubeIC& c, const PBuf& B, long long off, hipStream_t s) {
+    if (c.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tube_ic, grid1(c.n), dim3(kBS), 0, s, c, B, off);
     return hipGetLastError();
 }
 

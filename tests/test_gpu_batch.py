@@ -105,3 +105,18 @@ def test_failed_tube_fails_alone():
     lib.dsh_batch_status(h, 1, None, None, C.byref(steps), C.byref(done))
     assert (steps.value, done.value) == (0, 1)
     lib.dsh_batch_destroy(h)
+
+
+def test_device_setup_matches_host_setup():
+    """Batches generate their tubes on the device (k_tube_ic); before any step the state
+    equals the host set-up dsh_tube_ic (itself bit-exact against the oracle) bit for bit."""
+    from dustyshock3d_amd import sim
+    specs = [configs.cfg0(precision=abi.MIXED), configs.sweep_tube(3), configs.sweep_tube(11), configs.cfg1()]
+    b = Batch(specs)
+    for i, spec in enumerate(specs):
+        for sp in (0, 1):
+            host = sim.tube_ic(spec, sp)
+            dev = b.download(i, sp)
+            assert len(dev["x"]) == len(host["x"])
+            for k in ("x", "y", "z", "vx", "vy", "vz", "m", "rho", "p", "e", "h", "t_stop", "ind"):
+                assert np.array_equal(dev[k], host[k]), (i, sp, k)
