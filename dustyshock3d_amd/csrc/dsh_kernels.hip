@@ -773,16 +773,20 @@ __device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double y
     return dd < hs * hs ? 1u : 0u;
 }
 
-// The position arrays carry one window of padding: a window reads base + t, t < 64, with
-// base clamped to the staged count, and bits past a lane's own range are masked.  Separate
-// arrays (not one record per candidate) keep the lanes' 8-byte reads on distinct banks.
+// The density pass pads the position arrays by one window: a window reads base + t, t < 64,
+// with base clamped to the staged count, and bits past a lane's own range are masked.  The
+// force pass reads the density pass's hit masks; its rare fallback (more windows than mask
+// slots) stays inside each lane's range, so it needs no padding.  Separate arrays (not one
+// record per candidate) keep the lanes' 8-byte reads on distinct banks.  Velocities are
+// staged in the pair-arithmetic type R: v_i - v_j is formed in R, antisymmetric bit for bit.
 template <typename R, int KIND, bool kStats>
 struct TileLds {
-    double x[kTileP + 64], y[kTileP + 64], z[kTileP + 64], h[kTileP + 64];
-    double vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
+    static constexpr int kPad = KIND == 0 ? 64 : 0;
+    double x[kTileP + kPad], y[kTileP + kPad], z[kTileP + kPad], h[kTileP + kPad];
+    R vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
     R m[kTileP];
     R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
-    int ind[KIND ? kTileP : 1];
+    unsigned char ind[KIND ? kTileP : 1];
     int id[kStats ? kTileP : 1];
 };
 
@@ -866,6 +870,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
         mi_d = B.m[wo + q];
     }
     const R rhoi = (R)rhoi_d, pr2i = (R)pr2i_d, csi = (R)csi_d, mi = (R)mi_d;
+    const R vxi_r = (R)vxi, vyi_r = (R)vyi, vzi_r = (R)vzi;
     const R alpha = (R)T.alpha, beta = (R)T.beta, epsv = (R)T.eps;
     const bool plain = T.nsp == 1;   // gas-only run: de/dt of the plain ForceKernel
 
@@ -923,13 +928,13 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                         L.h[k] = B.h[g];
                         L.m[k] = (R)B.m[g];
                         if (KIND == 1) {
-                            L.vx[k] = B.vx[g];
-                            L.vy[k] = B.vy[g];
-                            L.vz[k] = B.vz[g];
+                            L.vx[k] = (R)B.vx[g];
+                            L.vy[k] = (R)B.vy[g];
+                            L.vz[k] = (R)B.vz[g];
                             L.rho[k] = (R)B.rho[g];
                             L.pr2[k] = (R)pr2[pb + k];
                             L.cs[k] = (R)csav[pb + k];
-                            L.ind[k] = B.ind[g];
+                            L.ind[k] = B.ind[g] != 0 ? 1 : 0;
                         }
                         if (kStats) L.id[k] = B.id[g];
                     }
@@ -943,9 +948,19 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                         unsigned long long mask = 0ull;
                         if (KIND == 1 && use_masks && slot < kMaskSlots) {
                             mask = mrow[(size_t)slot * 64];
+                        } else if (KIND == 1) {
+                            // no stored mask: each lane tests its own candidates only
+                            const int nt = min(max(len - w0, 0), 64);
+                            for (int t = 0; t < nt; ++t) {
+                                const int u = a + w0 + t;
+                                const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
+                                const uint32_t hit = shifted ? pred1_shift(c, xi, yi, zi, hi, shy, shz)
+                                                             : pred1(c, xi, yi, zi, hi);
+                                mask |= (unsigned long long)hit << t;
+                            }
                         } else {
-                            // uniform trip count: every lane evaluates wn candidates (index
-                            // clamped into the staged piece); bits past the lane's own range
+                            // uniform trip count: every lane evaluates wn candidates from a
+                            // base clamped into the padded piece; bits past the lane's own range
                             // are cleared once per window
                             const int rem = len - w0;
                             const unsigned long long valid =
@@ -997,8 +1012,8 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                     // momentum it injects exact to rounding
                                     const long long g = wo + pb + u;
                                     double coef, edc;
-                                    force_terms<double>(sqrt(dd), hhd, dx, dy, dz, vxi - L.vx[u], vyi - L.vy[u],
-                                                        vzi - L.vz[u], rhoi_d, B.rho[g], pr2i_d, pr2[pb + u], csi_d,
+                                    force_terms<double>(sqrt(dd), hhd, dx, dy, dz, vxi - B.vx[g], vyi - B.vy[g],
+                                                        vzi - B.vz[g], rhoi_d, B.rho[g], pr2i_d, pr2[pb + u], csi_d,
                                                         csav[pb + u], mi_d, B.m[g], T.alpha, T.beta, T.eps, plain,
                                                         coef, edc);
                                     d0 += dx * coef;
@@ -1008,8 +1023,8 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                 } else {
                                     R coef, edc;
                                     const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
-                                    force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, (R)(vxi - L.vx[u]),
-                                                   (R)(vyi - L.vy[u]), (R)(vzi - L.vz[u]), rhoi, L.rho[u], pr2i,
+                                    force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, vxi_r - L.vx[u],
+                                                   vyi_r - L.vy[u], vzi_r - L.vz[u], rhoi, L.rho[u], pr2i,
                                                    L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha, beta, epsv, plain, coef,
                                                    edc);
                                     a0.add(fx * coef);
