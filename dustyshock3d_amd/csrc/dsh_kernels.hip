@@ -1130,6 +1130,28 @@ __device__ __forceinline__ void idic_solve(int nd, double vg, const double* vd, 
     vdh[1] = d1;
 }
 
+// Visits the particles of species s in averaging cell (ax, ay, az): its f*f rows of f
+// neighbour cells are contiguous runs of the sorted order, walked as one run so that the 64
+// lanes stay busy across row ends.  fn(q) gets the sorted index.
+template <class F>
+__device__ __forceinline__ void for_avg_cell(const TubeConst& T, const int* __restrict__ start, int s, int ax,
+                                             int ay, int az, int lane, F&& fn) {
+    const int f = T.f, nx = T.n[0], ny = T.n[1], nr = f * f;
+    auto row_lo = [&](int r) { return T.cell_off[s] + ax * f + nx * ((ay * f + r % f) + ny * (az * f + r / f)); };
+    int r = 0, c0 = row_lo(0);
+    int lo = start[c0], len = start[c0 + f] - lo, off = 0;
+    for (int k = lane;; k += 64) {
+        while (k - off >= len) {
+            off += len;
+            if (++r >= nr) return;
+            c0 = row_lo(r);
+            lo = start[c0];
+            len = start[c0 + f] - lo;
+        }
+        fn(lo + (k - off));
+    }
+}
+
 // One wave per averaging cell: sums over its f^3 neighbour cells for every species,
 // implicit drag solve, then the per-particle kick (FP64), drift, energy and h update,
 // and the cell's conservation / CFL partials.  A gas-only tube takes the explicit kick
@@ -1147,7 +1169,6 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     if (D.done || D.err) return;
     const TubeConst& T = tc[t];
     const long long wo = (long long)(1 - *cur) * B.ntot;
-    const int f = T.f, nx = T.n[0], ny = T.n[1];
     const int ax_ = la % T.na[0];
     const int ay_ = (la / T.na[0]) % T.na[1];
     const int az_ = la / (T.na[0] * T.na[1]);
@@ -1160,37 +1181,32 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     int ndc[kMaxDust] = {0, 0};
     double sd[kMaxDust][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     for (int s = 0; s < T.nsp; ++s) {
-        for (int r = 0; r < f * f; ++r) {
-            const int cy = ay_ * f + r % f, cz = az_ * f + r / f;
-            const int c0 = T.cell_off[s] + ax_ * f + nx * (cy + ny * cz);
-            const int lo = start[c0], hi = start[c0 + f];
-            for (int k = lo + lane; k < hi; k += 64) {
-                if (s == 0) {
-                    ++ng;
-                    sg[0] += B.vx[wo + k];
-                    sg[1] += B.vy[wo + k];
-                    sg[2] += B.vz[wo + k];
-                    sg[3] += B.ax[wo + k];
-                    sg[4] += B.ay[wo + k];
-                    sg[5] += B.az[wo + k];
-                } else {
-                    ndc[s - 1]++;
-                    sd[s - 1][0] += B.vx[wo + k];
-                    sd[s - 1][1] += B.vy[wo + k];
-                    sd[s - 1][2] += B.vz[wo + k];
-                    double ts = B.ts[wo + k];
-                    if (T.drag_mode == 0 && B.ind[wo + k] == 0) {
-                        // t_stop_Kernel_IDIC (IDIC.h:78-91): the gas particle with the same
-                        // index, this step's pressure and density
-                        const long long gq = wo + ginv[T.p_off[0] + B.id[wo + k]];
-                        const double rg = B.rho[gq];
-                        ts = T.R[s - 1] / (sqrt(T.gamma * B.p[gq] / rg) * rg);
-                        B.ts[wo + k] = ts;
-                    }
-                    sd[s - 1][3] += ts;
+        for_avg_cell(T, start, s, ax_, ay_, az_, lane, [&](int k) {
+            if (s == 0) {
+                ++ng;
+                sg[0] += B.vx[wo + k];
+                sg[1] += B.vy[wo + k];
+                sg[2] += B.vz[wo + k];
+                sg[3] += B.ax[wo + k];
+                sg[4] += B.ay[wo + k];
+                sg[5] += B.az[wo + k];
+            } else {
+                ndc[s - 1]++;
+                sd[s - 1][0] += B.vx[wo + k];
+                sd[s - 1][1] += B.vy[wo + k];
+                sd[s - 1][2] += B.vz[wo + k];
+                double ts = B.ts[wo + k];
+                if (T.drag_mode == 0 && B.ind[wo + k] == 0) {
+                    // t_stop_Kernel_IDIC (IDIC.h:78-91): the gas particle with the same
+                    // index, this step's pressure and density
+                    const long long gq = wo + ginv[T.p_off[0] + B.id[wo + k]];
+                    const double rg = B.rho[gq];
+                    ts = T.R[s - 1] / (sqrt(T.gamma * B.p[gq] / rg) * rg);
+                    B.ts[wo + k] = ts;
                 }
+                sd[s - 1][3] += ts;
             }
-        }
+        });
     }
     ng = wave_sum_i(ng);
     for (int a = 0; a < 6; ++a) sg[a] = wave_sum(sg[a]);
@@ -1234,64 +1250,59 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     // per-particle update
     double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
     for (int s = 0; s < T.nsp; ++s) {
-        for (int r = 0; r < f * f; ++r) {
-            const int cy = ay_ * f + r % f, cz = az_ * f + r / f;
-            const int c0 = T.cell_off[s] + ax_ * f + nx * (cy + ny * cz);
-            const int lo = start[c0], hi = start[c0 + f];
-            for (int k = lo + lane; k < hi; k += 64) {
-                const long long g = wo + k;
-                double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
-                const double m = B.m[g];
-                double e = 0.0;
-                if (B.ind[g] == 0) {
-                    if (s == 0 && nd == 0) {
-                        // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
-                        vx = vx + B.ax[g] * tau;
-                        vy = vy + B.ay[g] * tau;
-                        vz = vz + B.az[g] * tau;
-                    } else if (s == 0) {
-                        vx = (vx / tau + tmpv[0] + psi[0]) / (1.0 / tau + tmp1);
-                        vy = (vy / tau + tmpv[1] + psi[1]) / (1.0 / tau + tmp1);
-                        vz = (vz / tau + tmpv[2] + psi[2]) / (1.0 / tau + tmp1);
-                    } else {
-                        const double tk = tb[s - 1];
-                        vx = (vx / tau + vgh[0] / tk) / (1.0 / tau + 1.0 / tk);
-                        vy = (vy / tau + vgh[1] / tk) / (1.0 / tau + 1.0 / tk);
-                        vz = (vz / tau + vgh[2] / tk) / (1.0 / tau + 1.0 / tk);
-                    }
-                    B.vx[g] = vx;
-                    B.vy[g] = vy;
-                    B.vz[g] = vz;
-                    double x = B.x[g] + vx * tau;
-                    double y = B.y[g] + vy * tau;
-                    double z = B.z[g] + vz * tau;
-                    if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
-                    if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
-                    B.x[g] = x;
-                    B.y[g] = y;
-                    B.z[g] = z;
-                    if (s == 0) {
-                        e = B.e[g] + B.ed[g] * tau;
-                        B.e[g] = e;
-                    }
-                    double h = B.h[g];
-                    if (T.h_mode == 1) {
-                        h = T.eta * cbrt(m / B.rho[g]);
-                        B.h[g] = h;
-                    }
-                    const double v = sqrt(vx * vx + vy * vy + vz * vz);
-                    mn = fmin(mn, h / (T.cs + v));
+        for_avg_cell(T, start, s, ax_, ay_, az_, lane, [&](int k) {
+            const long long g = wo + k;
+            double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
+            const double m = B.m[g];
+            double e = 0.0;
+            if (B.ind[g] == 0) {
+                if (s == 0 && nd == 0) {
+                    // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
+                    vx = vx + B.ax[g] * tau;
+                    vy = vy + B.ay[g] * tau;
+                    vz = vz + B.az[g] * tau;
                 } else if (s == 0) {
-                    e = B.e[g];
+                    vx = (vx / tau + tmpv[0] + psi[0]) / (1.0 / tau + tmp1);
+                    vy = (vy / tau + tmpv[1] + psi[1]) / (1.0 / tau + tmp1);
+                    vz = (vz / tau + tmpv[2] + psi[2]) / (1.0 / tau + tmp1);
+                } else {
+                    const double tk = tb[s - 1];
+                    vx = (vx / tau + vgh[0] / tk) / (1.0 / tau + 1.0 / tk);
+                    vy = (vy / tau + vgh[1] / tk) / (1.0 / tau + 1.0 / tk);
+                    vz = (vz / tau + vgh[2] / tk) / (1.0 / tau + 1.0 / tk);
                 }
-                px += m * vx;
-                py += m * vy;
-                pz += m * vz;
-                double kin = 0.5 * m * (vx * vx + vy * vy + vz * vz);
-                if (s == 0) kin += m * e;
-                E += kin;
+                B.vx[g] = vx;
+                B.vy[g] = vy;
+                B.vz[g] = vz;
+                double x = B.x[g] + vx * tau;
+                double y = B.y[g] + vy * tau;
+                double z = B.z[g] + vz * tau;
+                if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
+                if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
+                B.x[g] = x;
+                B.y[g] = y;
+                B.z[g] = z;
+                if (s == 0) {
+                    e = B.e[g] + B.ed[g] * tau;
+                    B.e[g] = e;
+                }
+                double h = B.h[g];
+                if (T.h_mode == 1) {
+                    h = T.eta * cbrt(m / B.rho[g]);
+                    B.h[g] = h;
+                }
+                const double v = sqrt(vx * vx + vy * vy + vz * vz);
+                mn = fmin(mn, h / (T.cs + v));
+            } else if (s == 0) {
+                e = B.e[g];
             }
-        }
+            px += m * vx;
+            py += m * vy;
+            pz += m * vz;
+            double kin = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+            if (s == 0) kin += m * e;
+            E += kin;
+        });
     }
     px = wave_sum(px);
     py = wave_sum(py);
