@@ -101,6 +101,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default piece size (192)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -113,6 +114,10 @@ def main():
     sim = Simulation.from_spec(spec)
     sim.set_pair_variant(args.variant)
     n_gas, n_dust = sim.counts[0], sim.counts[1]
+    tune = None
+    if args.variant == 1 and not args.no_autotune:
+        # untimed: each piece size runs 1 + 2 ordinary steps of the tube, the fastest is kept
+        tune = sim.autotune(steps_per_trial=2)
     sim.step(args.warmup)
     setup_s = time.perf_counter() - t_setup
     st0 = sim.status()
@@ -166,7 +171,8 @@ def main():
                                f"y,z width {spec.y_width}, dx {spec.dx_left}/{2 * spec.dx_left}, K={spec.K}, CFL 0.3",
                    "particles": n_gas + n_dust, "averaging_cells": n_avg, "t_at_start": st0["t"],
                    "dt": st1["dt"], "setup_s": round(setup_s, 2),
-                   "pair_kernels": "tiled" if args.variant == 1 else "direct"},
+                   "pair_kernels": "tiled" if args.variant == 1 else "direct",
+                   "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.config, dominant) if args.variant == 1 else None,

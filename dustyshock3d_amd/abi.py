@@ -174,6 +174,10 @@ def _declare(lib):
         "dsh_status": (C.c_int, [vp, _DP, _DP, _IP, _IP]),
         "dsh_grid": (C.c_int, [vp, _DP, _IP]),
         "dsh_set_stats": (C.c_int, [vp, C.c_int]),
+        "dsh_set_tile": (C.c_int, [vp, C.c_int]),
+        "dsh_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_double)]),
+        "dsh_batch_set_tile": (C.c_int, [vp, C.c_int]),
+        "dsh_batch_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_double)]),
         "dsh_set_pair_variant": (C.c_int, [vp, C.c_int]),
         "dsh_neighbour_stats": (C.c_int, [vp, C.c_int, _IP, P(C.c_uint64)]),
         "dsh_set_timing": (C.c_int, [vp, C.c_int]),

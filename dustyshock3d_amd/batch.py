@@ -60,6 +60,17 @@ class Batch:
         self._call(self._lib.dsh_batch_conservation(self._h, tube, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def set_tile(self, tile: int):
+        """Candidates staged per LDS piece by the tiled pair kernels (128, 192, 256)."""
+        self._call(self._lib.dsh_batch_set_tile(self._h, int(tile)))
+
+    def autotune(self, steps_per_trial: int = 2, with_variants: bool = False) -> dict:
+        """Time each pair-kernel configuration on the live state and keep the fastest."""
+        chosen = (C.c_int * 2)()
+        ms = C.c_double(0.0)
+        self._call(self._lib.dsh_batch_autotune(self._h, int(steps_per_trial), int(with_variants), chosen, C.byref(ms)))
+        return {"tile": int(chosen[0]), "variant": int(chosen[1]), "ms_per_step": ms.value}
+
     def set_pair_variant(self, v: int):
         self._call(self._lib.dsh_batch_set_pair_variant(self._h, int(v)))
 

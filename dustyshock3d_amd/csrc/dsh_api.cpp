@@ -93,6 +93,23 @@ int dsh_set_pair_variant(dsh_ctx* c, int variant) {
     return DSH_OK;
 }
 
+int dsh_set_tile(dsh_ctx* c, int tile) {
+    if (!c || (tile != 128 && tile != 192 && tile != 256)) return DSH_ERR_ARG;
+    c->eng->set_tile(tile);
+    return DSH_OK;
+}
+
+int dsh_autotune(dsh_ctx* c, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step) {
+    if (!c || steps_per_trial < 1) return DSH_ERR_ARG;
+    int tile = 0, variant = 0;
+    const int rc = c->set(c->eng->autotune(steps_per_trial, with_variants != 0, &tile, &variant, ms_per_step));
+    if (chosen) {
+        chosen[0] = tile;
+        chosen[1] = variant;
+    }
+    return rc;
+}
+
 int dsh_set_stats(dsh_ctx* c, int on) {
     if (!c) return DSH_ERR_ARG;
     c->eng->set_stats(on != 0);

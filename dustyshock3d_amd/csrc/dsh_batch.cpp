@@ -114,6 +114,23 @@ int dsh_batch_set_pair_variant(dsh_batch* b, int variant) {
     return DSH_OK;
 }
 
+int dsh_batch_set_tile(dsh_batch* b, int tile) {
+    if (!b || (tile != 128 && tile != 192 && tile != 256)) return DSH_ERR_ARG;
+    b->eng->set_tile(tile);
+    return DSH_OK;
+}
+
+int dsh_batch_autotune(dsh_batch* b, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step) {
+    if (!b || steps_per_trial < 1) return DSH_ERR_ARG;
+    int tile = 0, variant = 0;
+    const int rc = b->set(b->eng->autotune(steps_per_trial, with_variants != 0, &tile, &variant, ms_per_step));
+    if (chosen) {
+        chosen[0] = tile;
+        chosen[1] = variant;
+    }
+    return rc;
+}
+
 int dsh_batch_set_timing(dsh_batch* b, int on) {
     if (!b) return DSH_ERR_ARG;
     b->eng->set_timing(on != 0);

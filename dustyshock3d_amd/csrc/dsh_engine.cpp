@@ -296,6 +296,7 @@ int Engine::prepare() {
     if ((rc = make_map(pseg_, 64, a_.pmap64))) return rc;
     if ((rc = make_map(gseg_, 64, a_.gmap64))) return rc;
     a_.variant = pair_variant_;
+    a_.tile = tile_;
     if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
     a_.ginv = nullptr;
     for (int t = 0; t < T; ++t)
@@ -567,6 +568,57 @@ int Engine::run(int nsteps) {
         if ((rc = launch_steps(0))) return rc;
         if ((rc = sync())) return rc;
     }
+    return error_status();
+}
+
+int Engine::autotune(int steps_per_trial, bool with_variants, int* tile_out, int* variant_out, double* ms_out) {
+    if (steps_per_trial < 1) return fail(DSH_ERR_ARG, "steps_per_trial must be positive");
+    int rc;
+    if (!prepared_ && (rc = prepare())) return rc;
+    struct Cand {
+        int tile, variant;
+    };
+    std::vector<Cand> cands = {{128, kPairTiled}, {192, kPairTiled}, {256, kPairTiled}};
+    if (with_variants) cands.push_back({192, kPairDirect});
+    auto finished = [&]() {
+        for (const TubeDyn& d : dyn_h_)
+            if (!(d.done || d.err)) return false;
+        return true;
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((rc = hip_check(hipEventCreate(&e0), "hipEventCreate"))) return rc;
+    if ((rc = hip_check(hipEventCreate(&e1), "hipEventCreate"))) {
+        (void)hipEventDestroy(e0);
+        return rc;
+    }
+    Cand best{tile_, pair_variant_};
+    double best_ms = -1.0;
+    for (const Cand& c : cands) {
+        set_tile(c.tile);
+        set_pair_variant(c.variant);
+        if ((rc = run(1))) break;   // builds the graph, warms the caches
+        if (finished()) break;
+        if ((rc = hip_check(hipEventRecord(e0, stream_), "hipEventRecord"))) break;
+        if ((rc = launch_steps(steps_per_trial))) break;
+        if ((rc = hip_check(hipEventRecord(e1, stream_), "hipEventRecord"))) break;
+        if ((rc = sync())) break;
+        float ms = 0.f;
+        if ((rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime"))) break;
+        const double per = (double)ms / steps_per_trial;
+        if (best_ms < 0.0 || per < best_ms) {
+            best_ms = per;
+            best = c;
+        }
+        if (finished()) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_tile(best.tile);
+    set_pair_variant(best.variant);
+    if (rc) return rc;
+    if (tile_out) *tile_out = best.tile;
+    if (variant_out) *variant_out = best.variant;
+    if (ms_out) *ms_out = best_ms;
     return error_status();
 }
 

@@ -65,6 +65,18 @@ public:
         graph_dirty_ = true;
     }
     int pair_variant() const { return pair_variant_; }
+    // candidates staged per piece in the tiled pair kernels (128, 192 or 256); every size
+    // gives the same results bit for bit
+    void set_tile(int t) {
+        tile_ = t;
+        a_.tile = t;
+        graph_dirty_ = true;
+    }
+    int tile() const { return tile_; }
+    // Times steps_per_trial steps of each candidate configuration (piece sizes; with
+    // variants also the direct pair kernels, whose sums differ in rounding order) on the
+    // live state and keeps the fastest.  The trial steps are ordinary steps of the run.
+    int autotune(int steps_per_trial, bool with_variants, int* tile, int* variant, double* ms_per_step);
     // per-stage GPU time (ms) averaged over the steps launched with timing on
     const std::vector<std::pair<std::string, double>>& kernel_times() const { return ktimes_; }
     int timed_steps() const { return timed_steps_; }
@@ -97,6 +109,7 @@ private:
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
     int precision_ = 0;
     int pair_variant_ = kPairTiled;
+    int tile_ = 192;
     long long ntot_ = 0;
     size_t dev_bytes_ = 0;
     std::vector<std::pair<std::string, double>> ktimes_;

@@ -47,6 +47,7 @@ struct StepArgs {
     unsigned long long* masks;   // gas density hit masks reused by the force pass
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
     int variant;
+    int tile;              // candidates staged per piece in the tiled pair kernels: 128, 192, 256
 };
 
 // scratch: 5 doubles per particle block of pmap

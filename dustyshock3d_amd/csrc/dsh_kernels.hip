@@ -726,7 +726,8 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
 // 64 candidates, and does the pair arithmetic for the hits alone.  The candidate set of
 // a lane is the one of the direct kernels (cells within ceil(reach / c_a) of its own),
 // so the neighbour sets are identical.
-constexpr int kTileP = 192;     // candidates staged per piece
+// candidates staged per piece: a template parameter (128, 192 or 256) chosen at run time;
+// the pieces split each row's range in order, so every size gives the same sums bit for bit
 constexpr int kMaskSlots = 48;  // stored hit-mask words per lane of a gas chunk
 
 __device__ __forceinline__ int xcd_remap(int b, int nb) {
@@ -779,7 +780,7 @@ __device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double y
 // slots) stays inside each lane's range, so it needs no padding.  Separate arrays (not one
 // record per candidate) keep the lanes' 8-byte reads on distinct banks.  Velocities are
 // staged in the pair-arithmetic type R: v_i - v_j is formed in R, antisymmetric bit for bit.
-template <typename R, int KIND, bool kStats>
+template <typename R, int KIND, bool kStats, int kTileP>
 struct TileLds {
     static constexpr int kPad = KIND == 0 ? 64 : 0;
     double x[kTileP + kPad], y[kTileP + kPad], z[kTileP + kPad], h[kTileP + kPad];
@@ -818,7 +819,7 @@ __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dv
 // The gas density pass stores its predicate hit masks (one 64-bit word per lane per
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
 // the same iteration order and reads them instead of re-running the predicate.
-template <typename R, int KIND, bool kStats>
+template <typename R, int KIND, bool kStats, int kTileP>
 __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                                     PBuf B, const int* __restrict__ start,
@@ -826,7 +827,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                                                     double* __restrict__ pr2, double* __restrict__ csav,
                                                     int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
                                                     unsigned long long* __restrict__ masks, int mask_chunks) {
-    __shared__ TileLds<R, KIND, kStats> L;
+    __shared__ TileLds<R, KIND, kStats, kTileP> L;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int sgi = find_seg(mp, b);
     const Seg sg = mp.seg[sgi];
@@ -1478,10 +1479,19 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <typename R, int KIND, bool kStats, int kTileP>
+static void launch_tiled_t(const StepArgs& a, const Map& m, hipStream_t s) {
+    hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
+                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks);
+}
+
 template <typename R, int KIND, bool kStats>
 static void launch_tiled(const StepArgs& a, const Map& m, hipStream_t s) {
-    hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur, a.B,
-                       a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks);
+    switch (a.tile) {
+        case 128: launch_tiled_t<R, KIND, kStats, 128>(a, m, s); break;
+        case 256: launch_tiled_t<R, KIND, kStats, 256>(a, m, s); break;
+        default: launch_tiled_t<R, KIND, kStats, 192>(a, m, s); break;
+    }
 }
 
 hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s) {

@@ -155,6 +155,17 @@ class Simulation:
         return {"origin": d[0:3], "cell": d[3:6], "length": d[6:9], "n": i[0:3], "f": int(i[3]),
                 "n_avg": int(i[4])}
 
+    def set_tile(self, tile: int):
+        """Candidates staged per LDS piece by the tiled pair kernels (128, 192, 256)."""
+        self._call(self._lib.dsh_set_tile(self._ctx, int(tile)))
+
+    def autotune(self, steps_per_trial: int = 2, with_variants: bool = False) -> dict:
+        """Time each pair-kernel configuration on the live state and keep the fastest."""
+        chosen = (C.c_int * 2)()
+        ms = C.c_double(0.0)
+        self._call(self._lib.dsh_autotune(self._ctx, int(steps_per_trial), int(with_variants), chosen, C.byref(ms)))
+        return {"tile": int(chosen[0]), "variant": int(chosen[1]), "ms_per_step": ms.value}
+
     def set_pair_variant(self, variant: int):
         self._call(self._lib.dsh_set_pair_variant(self._ctx, int(variant)))
 

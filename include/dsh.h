@@ -178,6 +178,14 @@ int dsh_set_stats(dsh_ctx* ctx, int on);
 #define DSH_PAIR_DIRECT 0
 #define DSH_PAIR_TILED 1
 int dsh_set_pair_variant(dsh_ctx* ctx, int variant);
+/* Candidates staged per LDS piece by the tiled pair kernels: 128, 192 (default) or 256.
+   Pieces split each candidate row in order, so every size gives the same results bit for bit. */
+int dsh_set_tile(dsh_ctx* ctx, int tile);
+/* Autotuner (no reference counterpart): runs steps_per_trial steps with each piece size
+   (and, with with_variants, the direct pair kernels, whose sums round in another order)
+   on the live state, keeps the fastest and returns chosen[0] = tile, chosen[1] = variant
+   and its time per step.  The trial steps are ordinary steps of the run. */
+int dsh_autotune(dsh_ctx* ctx, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step);
 int dsh_neighbour_stats(dsh_ctx* ctx, int species, int* count, uint64_t* hash);
 /* Per-kernel GPU time of the last timed step, ms (HIP events on the step stream). */
 int dsh_set_timing(dsh_ctx* ctx, int on);
@@ -190,7 +198,7 @@ void dsh_destroy(dsh_ctx* ctx);
 typedef struct dsh_batch dsh_batch;
 /* ntubes tubes with their own parameters; every tube must be uploaded before stepping */
 int dsh_batch_create(const dsh_params* params, int ntubes, dsh_batch** out);
-/* convenience: parameters and seeded set-up of every tube from its spec (host set-up + upload) */
+/* convenience: parameters and seeded set-up of every tube from its spec, generated on the device */
 int dsh_batch_from_specs(const dsh_tube_spec* specs, int ntubes, dsh_batch** out);
 int dsh_batch_upload(dsh_batch* b, int tube, int species, const dsh_soa_view* view, int pm);
 /* advance every unfinished tube by up to nsteps steps; *ndone = finished tubes */
@@ -199,6 +207,8 @@ int dsh_batch_status(dsh_batch* b, int tube, double* t, double* dt, int* steps, 
 int dsh_batch_download(dsh_batch* b, int tube, int species, const dsh_soa_view* view);
 int dsh_batch_conservation(dsh_batch* b, int tube, double out[4]);
 int dsh_batch_set_pair_variant(dsh_batch* b, int variant);
+int dsh_batch_set_tile(dsh_batch* b, int tile);
+int dsh_batch_autotune(dsh_batch* b, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step);
 int dsh_batch_set_timing(dsh_batch* b, int on);
 int dsh_batch_kernel_times(dsh_batch* b, double* ms, char* names, int max_kernels, int name_len);
 /* bytes of device memory the batch holds */
