@@ -298,10 +298,13 @@ __global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __rest
     }
 }
 
-__global__ __launch_bounds__(kBS) void k_begin(int* __restrict__ count, int ncells, TubeDyn* __restrict__ td,
-                                               int ntubes) {
+__global__ __launch_bounds__(kBS) void k_begin(int* __restrict__ count, double* __restrict__ cellh, int ncells,
+                                               TubeDyn* __restrict__ td, int ntubes) {
     const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i < ncells) count[i] = 0;
+    if (i < ncells) {
+        count[i] = 0;
+        cellh[i] = 0.0;
+    }
     if (i < ntubes) {
         for (int s = 0; s < kMaxSpecies; ++s) td[i].hmax_bits[s] = 0ull;
     }
@@ -475,22 +478,41 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
     wkey[dst] = c;
 }
 
+// Largest h per neighbour cell and per (tube, species), one thread per sorted particle:
+// keys ascend along a wave, so a segmented max over runs of equal keys leaves each run's
+// maximum on its last lane, which raises the cell's value (h > 0, so the FP64 bits order
+// like the values).  Cells without particles keep the 0 written by k_begin.
 __global__ __launch_bounds__(kBS) void k_cellmax(Map mp, const int* __restrict__ cur, PBuf B,
-                                                 const int* __restrict__ start, double* __restrict__ cellh,
+                                                 const int* __restrict__ wkey, double* __restrict__ cellh,
                                                  TubeDyn* __restrict__ td) {
+    __shared__ double wmax[kBS / 64];
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
-    double hm = 0.0;
-    if (li < ip.sg.n) {
-        const int c = ip.sg.off + li;
-        const long long wo = (long long)(1 - *cur) * B.ntot;
-        const int lo = start[c], hi = start[c + 1];
-        for (int k = lo; k < hi; ++k) hm = fmax(hm, B.h[wo + k]);
-        cellh[c] = hm;
+    const int lane = threadIdx.x & 63;
+    const bool valid = li < ip.sg.n;
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    const int q = ip.sg.off + li;
+    const int k = valid ? wkey[q] : -1;
+    double h = valid ? B.h[wo + q] : 0.0;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double ho = __shfl_up(h, o, 64);
+        const int ko = __shfl_up(k, o, 64);
+        if (lane >= o && ko == k) h = fmax(h, ho);
     }
-    hm = wave_max(hm);
-    if ((threadIdx.x & 63) == 0 && hm > 0.0)
-        atomicMax(&td[ip.sg.tube].hmax_bits[ip.sg.species], (unsigned long long)__double_as_longlong(hm));
+    const int kn = __shfl_down(k, 1, 64);
+    if (valid && (lane == 63 || kn != k))
+        atomicMax(reinterpret_cast<unsigned long long*>(cellh + k), (unsigned long long)__double_as_longlong(h));
+    const double m = wave_max(h);
+    if (lane == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bm = 0.0;
+        for (int w = 0; w < kBS / 64; ++w) bm = fmax(bm, wmax[w]);
+        unsigned long long* t = &td[ip.sg.tube].hmax_bits[ip.sg.species];
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(bm);
+        // most blocks do not raise the tube maximum: read before the atomic
+        if (bm > 0.0 && bits > __atomic_load_n(t, __ATOMIC_RELAXED)) atomicMax(t, bits);
+    }
 }
 
 // Reach of a cell: every neighbour j of a particle i of the cell has |x_i - x_j| < h_i + h_j
@@ -1439,7 +1461,7 @@ hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s) {
 
 hipError_t launch_begin(const StepArgs& a, hipStream_t s) {
     const int n = a.ncells > a.ntubes ? a.ncells : a.ntubes;
-    hipLaunchKernelGGL(k_begin, grid1(n), dim3(kBS), 0, s, a.count, a.ncells, a.td, a.ntubes);
+    hipLaunchKernelGGL(k_begin, grid1(n), dim3(kBS), 0, s, a.count, a.cellh, a.ncells, a.td, a.ntubes);
     return hipGetLastError();
 }
 
@@ -1469,8 +1491,7 @@ hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_cellmax(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_cellmax, dim3(a.cmap.nblocks), dim3(kBS), 0, s, a.cmap, a.cur, a.B, a.start, a.cellh,
-                       a.td);
+    hipLaunchKernelGGL(k_cellmax, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.wkey, a.cellh, a.td);
     return hipGetLastError();
 }
 
