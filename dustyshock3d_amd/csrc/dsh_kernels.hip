@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace dsh {
 namespace {
 
@@ -132,6 +134,17 @@ template <typename R> __device__ __forceinline__ R spline_dw(R q) {
     const R u = R(2) - q;
     const R outer = R(-0.75) * u * u;
     return q <= R(1) ? inner : (q <= R(2) ? outer : R(0));
+}
+
+// W polynomial for a neighbour (q = r/h <= 2 up to the last rounding of r): Horner form
+// with fused multiply-adds; the q > 2 zero of spline_w is not needed.
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <typename R> __device__ __forceinline__ R spline_w_hit(R q) {
+    const R inner = fma_r(q * q, fma_r(R(0.75), q, R(-1.5)), R(1));
+    const R u = R(2) - q;
+    const R outer = R(0.25) * (u * u) * u;
+    return q < R(1) ? inner : outer;
 }
 
 // dW of a signed coordinate difference, without the 1/(pi h^4) factor (kernel.cu:24-38: odd in u).
@@ -807,7 +820,9 @@ struct TileLds {
     static constexpr int kPad = KIND == 0 ? 64 : 0;
     double x[kTileP + kPad], y[kTileP + kPad], z[kTileP + kPad], h[kTileP + kPad];
     R vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
-    R m[kTileP];
+    R m[KIND ? kTileP : 1];
+    R mw[KIND ? 1 : kTileP];    // density: m / pi
+    R hh[KIND ? 1 : kTileP];    // density: h / 2
     R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
     unsigned char ind[KIND ? kTileP : 1];
     int id[kStats ? kTileP : 1];
@@ -896,6 +911,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
     const R vxi_r = (R)vxi, vyi_r = (R)vyi, vzi_r = (R)vzi;
     const R alpha = (R)T.alpha, beta = (R)T.beta, epsv = (R)T.eps;
     const bool plain = T.nsp == 1;   // gas-only run: de/dt of the plain ForceKernel
+    const R hhi = (R)(0.5 * hi);
 
     bool pending = active;
     while (true) {
@@ -949,7 +965,12 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                         L.y[k] = B.y[g];
                         L.z[k] = B.z[g];
                         L.h[k] = B.h[g];
-                        L.m[k] = (R)B.m[g];
+                        if (KIND == 0) {
+                            L.mw[k] = (R)(B.m[g] * kInvPi);
+                            L.hh[k] = (R)(0.5 * B.h[g]);
+                        } else {
+                            L.m[k] = (R)B.m[g];
+                        }
                         if (KIND == 1) {
                             L.vx[k] = (R)B.vx[g];
                             L.vy[k] = (R)B.vy[g];
@@ -1009,7 +1030,31 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                             mask = (((unsigned long long)mhi << 32) | mlo) & valid;
                         }
                         if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
-                        while (mask) {
+                        if (KIND == 0) {
+                            // W of each hit; rows without a periodic image skip the shifts
+                            auto hits = [&](auto shifted_tag) {
+                                constexpr bool kSh = decltype(shifted_tag)::value;
+                                while (mask) {
+                                    const int t = __ffsll((long long)mask) - 1;
+                                    mask &= mask - 1ull;
+                                    const int u = a + w0 + t;
+                                    const double dx = L.x[u] - xi;
+                                    const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
+                                    const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                                    const R r = sqrt_r<R>((R)dd);
+                                    const R ih = rcp<R>(hhi + L.hh[u]);
+                                    a0.add(L.mw[u] * (ih * ih * ih) * spline_w_hit<R>(r * ih));
+                                    if (kStats) {
+                                        ++cnt;
+                                        hash += pair_tag(L.id[u], sy, sz);
+                                    }
+                                }
+                            };
+                            if (shifted) hits(std::true_type{});
+                            else hits(std::false_type{});
+                        }
+                        while (KIND == 1 && mask) {
                             const int t = __ffsll((long long)mask) - 1;
                             mask &= mask - 1ull;
                             const int u = a + w0 + t;
@@ -1018,17 +1063,7 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                             const double dy = (c.y - yi) + shy;
                             const double dz = (c.z - zi) + shz;
                             const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                            if (KIND == 0) {
-                                const R r = sqrt_r<R>((R)dd);
-                                const R hh = (R)(0.5 * (hi + c.h));
-                                const R ih = rcp<R>(hh);
-                                const R w = (R)kInvPi * (ih * ih * ih) * spline_w<R>(r * ih);
-                                a0.add(L.m[u] * w);
-                                if (kStats) {
-                                    ++cnt;
-                                    hash += pair_tag(L.id[u], sy, sz);
-                                }
-                            } else if (dd > 0.0) {
+                            if (dd > 0.0) {
                                 const double hhd = 0.5 * (hi + c.h);
                                 if (sizeof(R) == 4 && L.ind[u] != 0) {
                                     // a frozen neighbour has no partner term: FP64 keeps the
