@@ -472,20 +472,24 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
     B.vz[wo + dst] = B.vz[so + src];
     B.m[wo + dst] = B.m[so + src];
     B.h[wo + dst] = B.h[so + src];
-    B.rho[wo + dst] = B.rho[so + src];
-    B.ind[wo + dst] = B.ind[so + src];
+    const int ind = B.ind[so + src];
+    B.ind[wo + dst] = ind;
     B.id[wo + dst] = my;
+    // density, pressure and stopping time are recomputed this step for active particles of
+    // running tubes; frozen particles and finished tubes carry theirs along
+    const bool keep = ind != 0 || td[ip.sg.tube].done || td[ip.sg.tube].err;
+    if (keep) B.rho[wo + dst] = B.rho[so + src];
     if (ip.sg.species == 0) {
         if (ginv) ginv[ip.sg.off + my] = dst;
         B.e[wo + dst] = B.e[so + src];
-        B.p[wo + dst] = B.p[so + src];
+        if (keep) B.p[wo + dst] = B.p[so + src];
         if (td[ip.sg.tube].done) {
             B.ax[wo + dst] = B.ax[so + src];
             B.ay[wo + dst] = B.ay[so + src];
             B.az[wo + dst] = B.az[so + src];
             B.ed[wo + dst] = B.ed[so + src];
         }
-    } else {
+    } else if (keep) {
         B.ts[wo + dst] = B.ts[so + src];
     }
     wkey[dst] = c;
@@ -1379,9 +1383,10 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
 
 // One block per tube: deterministic reduction of the averaging-cell partials, then time,
 // next step size and completion (the loop control of kernel.cu:1929-1932,2566 plus CFL).
-__global__ __launch_bounds__(kBS) void k_finalize(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                  const int* __restrict__ cur, const double* __restrict__ part) {
-    __shared__ double sh[5][kBS / 64];
+constexpr int kFinBS = 1024;   // threads of the per-tube reduction
+__global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                     const int* __restrict__ cur, const double* __restrict__ part) {
+    __shared__ double sh[5][kFinBS / 64];
     __shared__ int skip;
     const int t = blockIdx.x;
     const TubeConst& T = tc[t];
@@ -1392,7 +1397,7 @@ __global__ __launch_bounds__(kBS) void k_finalize(const TubeConst* __restrict__ 
     const bool failed = D.err != 0;
     double a[4] = {0, 0, 0, 0}, mn = 1e300;
     if (!failed) {
-        for (int i = threadIdx.x; i < T.navg; i += kBS) {
+        for (int i = threadIdx.x; i < T.navg; i += kFinBS) {
             const double* o = part + 5ll * (T.avg_off + i);
             for (int c = 0; c < 4; ++c) a[c] += o[c];
             mn = fmin(mn, o[4]);
@@ -1414,7 +1419,7 @@ __global__ __launch_bounds__(kBS) void k_finalize(const TubeConst* __restrict__ 
         return;
     }
     double s[4] = {0, 0, 0, 0}, m2 = 1e300;
-    for (int k = 0; k < kBS / 64; ++k) {
+    for (int k = 0; k < kFinBS / 64; ++k) {
         for (int c = 0; c < 4; ++c) s[c] += sh[c][k];
         m2 = fmin(m2, sh[4][k]);
     }
@@ -1606,7 +1611,7 @@ hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kBS), 0, s, a.tc, a.td, a.cur, a.part);
+    hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kFinBS), 0, s, a.tc, a.td, a.cur, a.part);
     return hipGetLastError();
 }
 
