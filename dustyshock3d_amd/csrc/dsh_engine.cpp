@@ -331,6 +331,7 @@ int Engine::prepare() {
     if ((rc = alloc(&a_.start, (size_t)coff + 1))) return rc;
     if ((rc = alloc(&a_.cellh, (size_t)coff))) return rc;
     if ((rc = alloc(&a_.reach, (size_t)coff))) return rc;
+    if ((rc = alloc(&a_.wtmp, 2 * (size_t)coff))) return rc;
     a_.scan_nb = scan_blocks((int)coff);
     if ((rc = alloc(&a_.scan_bsum, (size_t)a_.scan_nb))) return rc;
     if ((rc = alloc(&a_.part, 5 * (size_t)aoff))) return rc;

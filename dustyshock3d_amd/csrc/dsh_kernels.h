@@ -36,6 +36,7 @@ struct StepArgs {
     int *count, *start;
     double* cellh;
     double* reach;
+    double* wtmp;          // 2 x ncells: separable window maxima of cell h
     int ncells;
     int* scan_bsum;
     int scan_nb;

@@ -537,8 +537,61 @@ __global__ __launch_bounds__(kBS) void k_cellmax(Map mp, const int* __restrict__
 // axis are at least W_a c_a >= 2 h_max apart, so h_j is bounded by the max of cell h over
 // that window (a periodic axis shorter than the window is scanned once), and the pair
 // kernels search ceil(reach / c_a) cells along each axis.
+// Search window of the current step along each axis: W_a = ceil(2 h_max (1+m) / c_a).
+__device__ __forceinline__ bool radius_window(const TubeConst& T, const TubeDyn& D, int species, int w[3]) {
+    const double hs = __longlong_as_double((long long)D.hmax_bits[species]);
+    const double span = 2.0 * hs * (1.0 + kRadiusMargin);
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+        w[a] = (int)ceil(span * T.ic[a]);
+        ok = ok && w[a] <= kMaxCellReach;
+    }
+    return ok;
+}
+
+// One axis of the window maximum of cell h, separably: out[c] = max of in over the cells
+// within W_a of c along axis a (clamped on a bounded axis, wrapped on a periodic one; a
+// periodic axis shorter than the window is taken whole).  Max is exact, so three passes
+// give the same value as the full window.
+__global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __restrict__ tc,
+                                                    const TubeDyn* __restrict__ td, const double* __restrict__ in,
+                                                    double* __restrict__ out, int axis) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const TubeConst& T = tc[ip.sg.tube];
+    if (T.h_mode == 0) return;
+    int w[3];
+    if (!radius_window(T, td[ip.sg.tube], ip.sg.species, w)) return;
+    const int n0 = T.n[0], n1 = T.n[1];
+    const int k[3] = {li % n0, (li / n0) % n1, li / (n0 * n1)};
+    const int stride = axis == 0 ? 1 : (axis == 1 ? n0 : n0 * n1);
+    const int n = T.n[axis], ka = k[axis];
+    const int base = ip.sg.off + li - ka * stride;
+    double m = 0.0;
+    if (T.per[axis] && 2 * w[axis] + 1 >= n) {
+        for (int c = 0; c < n; ++c) m = fmax(m, in[base + c * stride]);
+    } else if (T.per[axis]) {
+        for (int o = -w[axis]; o <= w[axis]; ++o) {
+            int c = ka + o;
+            c -= floordiv(c, n) * n;
+            m = fmax(m, in[base + c * stride]);
+        }
+    } else {
+        const int c0 = max(ka - w[axis], 0), c1 = min(ka + w[axis], n - 1);
+        for (int c = c0; c <= c1; ++c) m = fmax(m, in[base + c * stride]);
+    }
+    out[ip.sg.off + li] = m;
+}
+
+// Reach of a cell: every neighbour j of a particle i of the cell has |x_i - x_j| < h_i + h_j
+// <= h_cell + h_j.  Cells more than W_a = ceil(2 h_max (1+m) / c_a) cells away along an
+// axis are at least W_a c_a >= 2 h_max apart, so h_j is bounded by the max of cell h over
+// that window (wmax, from the three k_window_max passes), and the pair kernels search
+// ceil(reach / c_a) cells along each axis.
 __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                const double* __restrict__ cellh, double* __restrict__ reach) {
+                                                const double* __restrict__ cellh, const double* __restrict__ wmax,
+                                                double* __restrict__ reach) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
@@ -550,51 +603,18 @@ __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restr
         reach[c] = 0.0;
         return;
     }
-    const double hs = __longlong_as_double((long long)D.hmax_bits[ip.sg.species]);
-    const double span = 2.0 * hs * (1.0 + kRadiusMargin);
     int w[3];
-    for (int a = 0; a < 3; ++a) {
-        w[a] = (int)ceil(span * T.ic[a]);
-        if (w[a] > kMaxCellReach) {
-            if (li == 0) atomicOr(&D.err, kErrGrid);
-            reach[c] = 0.0;
-            return;
-        }
-    }
-    if (T.h_mode == 0) {
-        reach[c] = span;
+    if (!radius_window(T, D, ip.sg.species, w)) {
+        if (li == 0) atomicOr(&D.err, kErrGrid);
+        reach[c] = 0.0;
         return;
     }
-    const int kx = li % T.n[0];
-    const int ky = (li / T.n[0]) % T.n[1];
-    const int kz = li / (T.n[0] * T.n[1]);
-    const int base = ip.sg.off;
-    // a periodic axis whose window covers every cell is scanned once
-    const int lz = (T.per[2] && 2 * w[2] + 1 >= T.n[2]) ? 0 : -w[2];
-    const int uz = (T.per[2] && 2 * w[2] + 1 >= T.n[2]) ? T.n[2] - 1 : w[2];
-    const int ly = (T.per[1] && 2 * w[1] + 1 >= T.n[1]) ? 0 : -w[1];
-    const int uy = (T.per[1] && 2 * w[1] + 1 >= T.n[1]) ? T.n[1] - 1 : w[1];
-    const bool absz = T.per[2] && 2 * w[2] + 1 >= T.n[2];
-    const bool absy = T.per[1] && 2 * w[1] + 1 >= T.n[1];
-    double wm = 0.0;
-    for (int oz = lz; oz <= uz; ++oz) {
-        int cz = absz ? oz : kz + oz;
-        if (!absz) {
-            if (T.per[2]) cz -= floordiv(cz, T.n[2]) * T.n[2];
-            else if (cz < 0 || cz >= T.n[2]) continue;
-        }
-        for (int oy = ly; oy <= uy; ++oy) {
-            int cy = absy ? oy : ky + oy;
-            if (!absy) {
-                if (T.per[1]) cy -= floordiv(cy, T.n[1]) * T.n[1];
-                else if (cy < 0 || cy >= T.n[1]) continue;
-            }
-            const int row = base + T.n[0] * (cy + T.n[1] * cz);
-            const int x0 = max(kx - w[0], 0), x1 = min(kx + w[0], T.n[0] - 1);
-            for (int cx = x0; cx <= x1; ++cx) wm = fmax(wm, cellh[row + cx]);
-        }
+    if (T.h_mode == 0) {
+        const double hs = __longlong_as_double((long long)D.hmax_bits[ip.sg.species]);
+        reach[c] = 2.0 * hs * (1.0 + kRadiusMargin);
+        return;
     }
-    reach[c] = (hc + wm) * (1.0 + kRadiusMargin);
+    reach[c] = (hc + wmax[c]) * (1.0 + kRadiusMargin);
 }
 
 template <typename R, bool kStats>
@@ -1536,7 +1556,11 @@ hipError_t launch_cellmax(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_radius, dim3(a.cmap.nblocks), dim3(kBS), 0, s, a.cmap, a.tc, a.td, a.cellh, a.reach);
+    const dim3 g(a.cmap.nblocks), b(kBS);
+    hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, 0);
+    hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp, a.wtmp + a.ncells, 1);
+    hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp + a.ncells, a.wtmp, 2);
+    hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, a.reach);
     return hipGetLastError();
 }
 
