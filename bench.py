@@ -53,22 +53,24 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
 
 
 # kernels behind each pass, as named in the rocprofv3 summaries under profiles/
-PASS_KERNELS = {
-    "density": ["k_pairs_tiled<float, 0, false>"],
-    "force": ["k_pairs_tiled<float, 1, false>"],
-    "idic": ["k_idic"],
-    "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
-             "k_cellmax", "k_radius"],
-}
+def pass_kernels(stage, tile=192):
+    """Kernel names (as rocprofv3 reports them) of one pass of the step."""
+    return {
+        "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
+        "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
+        "idic": ["k_idic"],
+        "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
+                 "k_cellmax", "k_window_max", "k_radius"],
+    }.get(stage, [])
 
 
-def measured_traffic(workload, stage):
+def measured_traffic(workload, stage, tile=192):
     """HBM bytes per launch of a pass from the committed FETCH_SIZE/WRITE_SIZE summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     kern = json.load(open(path))["kernels"]
-    names = PASS_KERNELS.get(stage, [])
+    names = pass_kernels(stage, tile)
     if not names or any(n not in kern for n in names):
         return None
     return sum(kern[n]["read_bytes"] + kern[n]["write_bytes"] for n in names)
@@ -175,7 +177,8 @@ def main():
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, dominant) if args.variant == 1 else None,
+                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else 192)
+                     if args.variant == 1 else None,
                      "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms},
         "step_roofline": {"algo_bytes": step_bytes, "event_ms": step_ms,

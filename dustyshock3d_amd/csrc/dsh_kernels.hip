@@ -147,6 +147,14 @@ template <typename R> __device__ __forceinline__ R spline_w_hit(R q) {
     return q < R(1) ? inner : outer;
 }
 
+// dW polynomial for a neighbour (q <= 2 up to the last rounding), Horner form, no branch.
+template <typename R> __device__ __forceinline__ R spline_dw_hit(R q) {
+    const R inner = q * fma_r(R(2.25), q, R(-3));
+    const R u = R(2) - q;
+    const R outer = R(-0.75) * (u * u);
+    return q <= R(1) ? inner : outer;
+}
+
 // dW of a signed coordinate difference, without the 1/(pi h^4) factor (kernel.cu:24-38: odd in u).
 template <typename R> __device__ __forceinline__ R spline_dw_signed(R u) {
     return u < R(0) ? -spline_dw<R>(-u) : spline_dw<R>(u);
@@ -867,7 +875,7 @@ __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dv
         const T mu = hh * nu * rcp<T>(dist * dist + epsv * hh * hh);
         fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
     }
-    const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw<T>(dist * ih);
+    const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw_hit<T>(dist * ih);
     const T dwr = dw * rcp<T>(dist);
     if (plain)
         edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
