@@ -1044,20 +1044,27 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
                             const int base = min(a + w0, pn);
                             // candidates are shifted in from the top, so candidate t lands on bit t
                             const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
+                            // whole groups of 4 (reads past wn stay in the padding; their bits
+                            // are masked)
+                            const int wn4 = (wn + 3) & ~3;
+                            const int n0 = min(wn4, 32);
                             uint32_t mlo = 0u, mhi = 0u;
-                            const int n0 = min(wn, 32);
+                            auto group = [&](auto shifted_tag, uint32_t m, int t) {
+                                constexpr bool kSh = decltype(shifted_tag)::value;
+#pragma unroll
+                                for (int k = 3; k >= 0; --k) {
+                                    const P4 c{cx_[t + k], cy_[t + k], cz_[t + k], ch_[t + k]};
+                                    m = (m << 1) | (kSh ? pred1_shift(c, xi, yi, zi, hi, shy, shz)
+                                                        : pred1(c, xi, yi, zi, hi));
+                                }
+                                return m;
+                            };
                             if (!shifted) {
-#pragma unroll 4
-                                for (int t = wn - 1; t >= 32; --t) mhi = (mhi << 1) | pred1(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi);
-#pragma unroll 4
-                                for (int t = n0 - 1; t >= 0; --t) mlo = (mlo << 1) | pred1(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi);
+                                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::false_type{}, mhi, t);
+                                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::false_type{}, mlo, t);
                             } else {
-#pragma unroll 4
-                                for (int t = wn - 1; t >= 32; --t)
-                                    mhi = (mhi << 1) | pred1_shift(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi, shy, shz);
-#pragma unroll 4
-                                for (int t = n0 - 1; t >= 0; --t)
-                                    mlo = (mlo << 1) | pred1_shift(P4{cx_[t], cy_[t], cz_[t], ch_[t]}, xi, yi, zi, hi, shy, shz);
+                                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::true_type{}, mhi, t);
+                                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::true_type{}, mlo, t);
                             }
                             mask = (((unsigned long long)mhi << 32) | mlo) & valid;
                         }
