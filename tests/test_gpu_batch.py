@@ -120,3 +120,40 @@ def test_device_setup_matches_host_setup():
             assert len(dev["x"]) == len(host["x"])
             for k in ("x", "y", "z", "vx", "vy", "vz", "m", "rho", "p", "e", "h", "t_stop", "ind"):
                 assert np.array_equal(dev[k], host[k]), (i, sp, k)
+
+
+def test_mixed_physics_batch_matches_alone():
+    """One batch holding a gas-only tube (plain scheme), a K-law dusty tube and a tube with the
+    reference's own stopping-time law and two dust species: each equals its solo run bit for bit."""
+    import ctypes as C
+    import os
+    from dustyshock3d_amd import sim
+    from tests.parity_util import inject
+    gas_spec = configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=6, tube_index=3)
+    p_gas = sim.tube_params(gas_spec)
+    p_gas.n_dust = 0
+    st_gas = [sim.tube_ic(gas_spec, 0)]
+    dusty_spec = configs.tube(0.02, 0.01, 30.0, 2.0, max_steps=6, tube_index=4)
+    p_dusty = sim.tube_params(dusty_spec)
+    st_dusty = [sim.tube_ic(dusty_spec, 0), sim.tube_ic(dusty_spec, 1)]
+    p_ref = sim.read_init(os.path.join(os.path.dirname(__file__), "golden", "Init.txt"))
+    p_ref.particles_per_length, p_ref.h, p_ref.max_steps = 20, 0.2, 6
+    sim.reference_setup(p_ref)
+    st_ref = [sim.reference_ic(p_ref, s) for s in range(3)]
+    params = [p_gas, p_dusty, p_ref]
+    states = [st_gas, st_dusty, st_ref]
+    lib, h = _raw_batch(states, params)
+    nd = C.c_int()
+    assert lib.dsh_batch_run(h, 50, C.byref(nd)) == abi.OK
+    assert nd.value == 3
+    for t, (p, st) in enumerate(zip(params, states)):
+        solo = inject(Simulation, p, {s: a for s, a in enumerate(st)})
+        solo.step(50)
+        for s in range(len(st)):
+            arrs = abi.empty_arrays(len(st[s]["x"]))
+            v = abi.make_view(arrs)
+            assert lib.dsh_batch_download(h, t, s, C.byref(v)) == abi.OK
+            ref = solo.download(s)
+            for k in FIELDS:
+                assert np.array_equal(arrs[k], ref[k]), (t, s, k)
+    lib.dsh_batch_destroy(h)
