@@ -815,6 +815,8 @@ __device__ __forceinline__ int wave_max_int(int v) {
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
     return v;
 }
+// a value equal on every lane, as a scalar (kept in SGPRs)
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // One candidate of the FP64 neighbour predicate, as a bit at position t (t < 32, uniform).
 // One candidate's position and smoothing length, gathered from the staged arrays.
@@ -889,7 +891,7 @@ __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dv
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
 // the same iteration order and reads them instead of re-running the predicate.
 template <typename R, int KIND, bool kStats, int kTileP>
-__global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                                     PBuf B, const int* __restrict__ start,
                                                     const int* __restrict__ wkey, const double* __restrict__ reach,
@@ -945,195 +947,288 @@ __global__ __launch_bounds__(64) void k_pairs_tiled(Map mp, const TubeConst* __r
     const bool plain = T.nsp == 1;   // gas-only run: de/dt of the plain ForceKernel
     const R hhi = (R)(0.5 * hi);
 
+    // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
+    // range [ub, ue) of all lanes, this lane's own range [jl, jh) (empty if it does not need
+    // the row), and the periodic image of the row.
+    struct RowSeg {
+        int ub, ue, jl, jh, sy, sz;
+        double shy, shz;
+        bool on, shifted;
+    };
+    // Stages the sorted candidates [s0, s0+n0) then [s1, s1+n1) into LDS slots [0, n0+n1).
+    // All loads of a candidate are issued before its LDS writes (the scheduling barrier
+    // keeps the compiler from serialising them under register pressure).
+    auto stage = [&](int s0, int n0, int s1, int n1) {
+        __syncthreads();
+        for (int k = lane; k < n0 + n1; k += 64) {
+            const int src = k < n0 ? s0 + k : s1 + (k - n0);
+            const long long g = wo + src;
+            const double x = B.x[g], y = B.y[g], z = B.z[g], h = B.h[g], m = B.m[g];
+            double vx = 0, vy = 0, vz = 0, rho = 0, p2 = 0, cs = 0;
+            int ind = 0, id = 0;
+            if (KIND == 1) {
+                vx = B.vx[g];
+                vy = B.vy[g];
+                vz = B.vz[g];
+                rho = B.rho[g];
+                p2 = pr2[src];
+                cs = csav[src];
+                ind = B.ind[g];
+            }
+            if (kStats) id = B.id[g];
+            __builtin_amdgcn_sched_barrier(0);
+            L.x[k] = x;
+            L.y[k] = y;
+            L.z[k] = z;
+            L.h[k] = h;
+            if (KIND == 0) {
+                L.mw[k] = (R)(m * kInvPi);
+                L.hh[k] = (R)(0.5 * h);
+            } else {
+                L.m[k] = (R)m;
+                L.vx[k] = (R)vx;
+                L.vy[k] = (R)vy;
+                L.vz[k] = (R)vz;
+                L.rho[k] = (R)rho;
+                L.pr2[k] = (R)p2;
+                L.cs[k] = (R)cs;
+                L.ind[k] = ind != 0 ? 1 : 0;
+            }
+            if (kStats) L.id[k] = id;
+        }
+        __syncthreads();
+    };
+    // Hit mask of one window of 64 candidates of this lane's range, starting at staged slot
+    // a (>= 0); len = candidates of the lane's range from a on, wn = the wave's maximum of
+    // len (<= 64 used), pn = staged count.  The force pass reads the density pass's mask.
+    auto window_mask = [&](int a, int len, int wn, int pn, const RowSeg& g) -> unsigned long long {
+        unsigned long long mask = 0ull;
+        if (KIND == 1 && use_masks && slot < kMaskSlots) {
+            mask = mrow[(size_t)slot * 64];
+        } else if (KIND == 1) {
+            // no stored mask: each lane tests its own candidates only
+            const int nt = min(max(len, 0), 64);
+            for (int t = 0; t < nt; ++t) {
+                const int u = a + t;
+                const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
+                const uint32_t hit = g.shifted ? pred1_shift(c, xi, yi, zi, hi, g.shy, g.shz)
+                                               : pred1(c, xi, yi, zi, hi);
+                mask |= (unsigned long long)hit << t;
+            }
+        } else {
+            // uniform trip count: every lane evaluates wn candidates from a base clamped into
+            // the padded piece; bits past the lane's own range are cleared once per window
+            wn = min(wn, 64);
+            const unsigned long long valid = len >= 64 ? ~0ull : (len <= 0 ? 0ull : ((1ull << len) - 1ull));
+            const int base = min(a, pn);
+            // candidates are shifted in from the top, so candidate t lands on bit t
+            const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
+            // whole groups of 4 (reads past wn stay in the padding; their bits are masked)
+            const int wn4 = (wn + 3) & ~3;
+            const int n0 = min(wn4, 32);
+            uint32_t mlo = 0u, mhi = 0u;
+            const double shy = g.shy, shz = g.shz;
+            auto group = [&](auto shifted_tag, uint32_t m, int t) {
+                constexpr bool kSh = decltype(shifted_tag)::value;
+#pragma unroll
+                for (int k = 3; k >= 0; --k) {
+                    const P4 c{cx_[t + k], cy_[t + k], cz_[t + k], ch_[t + k]};
+                    m = (m << 1) | (kSh ? pred1_shift(c, xi, yi, zi, hi, shy, shz) : pred1(c, xi, yi, zi, hi));
+                }
+                return m;
+            };
+            if (!g.shifted) {
+                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::false_type{}, mhi, t);
+                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::false_type{}, mlo, t);
+            } else {
+                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::true_type{}, mhi, t);
+                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::true_type{}, mlo, t);
+            }
+            mask = (((unsigned long long)mhi << 32) | mlo) & valid;
+        }
+        if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
+        ++slot;
+        return mask;
+    };
+    // Pair arithmetic of one hit at staged slot u; gi = the candidate's sorted index.
+    auto hit = [&](auto shifted_tag, int u, double shy, double shz, int sy, int sz, int gi) {
+        constexpr bool kSh = decltype(shifted_tag)::value;
+        const double dx = L.x[u] - xi;
+        const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
+        const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+        const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        if (KIND == 0) {
+            const R r = sqrt_r<R>((R)dd);
+            const R ih = rcp<R>(hhi + L.hh[u]);
+            a0.add(L.mw[u] * (ih * ih * ih) * spline_w_hit<R>(r * ih));
+            if (kStats) {
+                ++cnt;
+                hash += pair_tag(L.id[u], sy, sz);
+            }
+        } else if (dd > 0.0) {
+            const double hhd = 0.5 * (hi + L.h[u]);
+            if (sizeof(R) == 4 && L.ind[u] != 0) {
+                // a frozen neighbour has no partner term: FP64 keeps the momentum it
+                // injects exact to rounding
+                // (the target's FP64 values are read again here rather than kept in registers)
+                const long long g = wo + gi, gq = wo + q;
+                double coef, edc;
+                force_terms<double>(sqrt(dd), hhd, dx, dy, dz, B.vx[gq] - B.vx[g], B.vy[gq] - B.vy[g],
+                                    B.vz[gq] - B.vz[g], B.rho[gq], B.rho[g], pr2[q], pr2[gi], csav[q], csav[gi],
+                                    B.m[gq], B.m[g], T.alpha, T.beta, T.eps, plain, coef, edc);
+                d0 += dx * coef;
+                d1 += dy * coef;
+                d2s += dz * coef;
+                d3 += edc;
+            } else {
+                R coef, edc;
+                const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
+                force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
+                               vzi_r - L.vz[u], rhoi, L.rho[u], pr2i, L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha,
+                               beta, epsv, plain, coef, edc);
+                a0.add(fx * coef);
+                a1.add(fy * coef);
+                a2.add(fz * coef);
+                a3.add(edc);
+            }
+        }
+    };
+    // Hits of one window whose slot 0 is staged slot o and whose candidates start at sorted
+    // index gb + o.
+    auto hits1 = [&](auto shifted_tag, unsigned long long m, int o, int gb, const RowSeg& g) {
+        while (m) {
+            const int t = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            hit(shifted_tag, o + t, g.shy, g.shz, g.sy, g.sz, gb + o + t);
+        }
+    };
+    // Hits of two windows: a lane takes all of its hits of the first (candidates from sorted
+    // index gA + staged slot), then of the second (gB + slot), so the loop runs max over
+    // lanes of the total, not the sum of the two maxima.  The order per lane is the one of
+    // hits1(first); hits1(second).
+    auto hits2 = [&](auto shifted_tag, unsigned long long mA, int oA, int gA, const RowSeg& A,
+                     unsigned long long mB, int oB, int gB, const RowSeg& B2) {
+        bool second = mA == 0ull;
+        unsigned long long m = second ? mB : mA;
+        int o = second ? oB : oA;
+        while (m) {
+            const int t = __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            const int u = o + t;
+            hit(shifted_tag, u, second ? B2.shy : A.shy, second ? B2.shz : A.shz, second ? B2.sy : A.sy,
+                second ? B2.sz : A.sz, (second ? gB : gA) + u);
+            if (m == 0ull && !second) {
+                second = true;
+                m = mB;
+                o = oB;
+            }
+        }
+    };
+    // All pieces and windows of one row, in order.
+    auto run_row = [&](const RowSeg& g) {
+        for (int pb = g.ub; pb < g.ue; pb += kTileP) {
+            const int pn = min(kTileP, g.ue - pb);
+            stage(pb, pn, 0, 0);
+            const int a = max(g.jl, pb) - pb;
+            const int e = min(g.jh, pb + pn) - pb;
+            const int len = e > a ? e - a : 0;
+            const int maxlen = uni(wave_max_int(len));
+            for (int w0 = 0; w0 < maxlen; w0 += 64) {
+                const unsigned long long mask = window_mask(a + w0, len - w0, maxlen - w0, pn, g);
+                if (KIND == 1) {
+                    // one copy of the large force body: the image shift is added always
+                    // (+0.0 for a row without one, which leaves the difference unchanged)
+                    hits2(std::true_type{}, mask, a + w0, pb, g, 0ull, 0, pb, g);
+                } else if (g.shifted) {
+                    hits1(std::true_type{}, mask, a + w0, pb, g);
+                } else {
+                    hits1(std::false_type{}, mask, a + w0, pb, g);
+                }
+            }
+        }
+    };
+    // Two rows staged as one piece when both fit and every lane's range in each is one
+    // window; same windows, slots and per-lane order as run_row(A); run_row(B).
+    auto run_pair = [&](const RowSeg& A, const RowSeg& B2) -> bool {
+        const int nA = A.ue - A.ub, nB = B2.ue - B2.ub;
+        if (nA + nB > kTileP) return false;
+        const int lenA = A.jh > A.jl ? A.jh - A.jl : 0;
+        const int lenB = B2.jh > B2.jl ? B2.jh - B2.jl : 0;
+        const int mlA = uni(wave_max_int(lenA)), mlB = uni(wave_max_int(lenB));
+        if (mlA > 64 || mlB > 64) return false;
+        stage(A.ub, nA, B2.ub, nB);
+        const int aA = max(A.jl, A.ub) - A.ub;
+        const int aB = nA + max(B2.jl, B2.ub) - B2.ub;
+        const unsigned long long mA = mlA > 0 ? window_mask(aA, lenA, mlA, nA + nB, A) : 0ull;
+        const unsigned long long mB = mlB > 0 ? window_mask(aB, lenB, mlB, nA + nB, B2) : 0ull;
+        if (KIND == 1 || A.shifted || B2.shifted)
+            hits2(std::true_type{}, mA, aA, A.ub, A, mB, aB, B2.ub - nA, B2);
+        else
+            hits2(std::false_type{}, mA, aA, A.ub, A, mB, aB, B2.ub - nA, B2);
+        return true;
+    };
+
     bool pending = active;
     while (true) {
         const unsigned long long pm = __ballot(pending);
         if (pm == 0ull) break;
         const int leader = __ffsll((long long)pm) - 1;
-        const int row0 = __shfl(row, leader, 64);
+        const int row0 = uni(__shfl(row, leader, 64));
         const bool mine = pending && row == row0;
-        const int cxa = wave_min_int(mine ? kx : 0x7fffffff);
-        const int cxb = wave_max_int(mine ? kx : -1);
+        const int cxa = uni(wave_min_int(mine ? kx : 0x7fffffff));
+        const int cxb = uni(wave_max_int(mine ? kx : -1));
         const double rmax = wave_max(mine ? rch : 0.0);
-        const int Rx = (int)ceil(rmax * T.ic[0]);
-        const int Ry = (int)ceil(rmax * T.ic[1]);
-        const int Rz = (int)ceil(rmax * T.ic[2]);
+        const int Rx = uni((int)ceil(rmax * T.ic[0]));
+        const int Ry = uni((int)ceil(rmax * T.ic[1]));
+        const int Rz = uni((int)ceil(rmax * T.ic[2]));
         const int ky0 = row0 % ny, kz0 = row0 / ny;
-        for (int oz = -Rz; oz <= Rz; ++oz) {
+        const int nry = 2 * Ry + 1, nrows = nry * (2 * Rz + 1);
+        auto row_seg = [&](int r) {
+            RowSeg g{};
+            g.on = false;
+            const int oy = r % nry - Ry, oz = r / nry - Rz;
             int cz = kz0 + oz, sz = 0;
             if (T.per[2]) {
                 sz = floordiv(cz, nz);
                 cz -= sz * nz;
             } else if (cz < 0 || cz >= nz) {
-                continue;
+                return g;
             }
-            const double shz = (double)sz * T.L[2];
-            for (int oy = -Ry; oy <= Ry; ++oy) {
-                int cy = ky0 + oy, sy = 0;
-                if (T.per[1]) {
-                    sy = floordiv(cy, ny);
-                    cy -= sy * ny;
-                } else if (cy < 0 || cy >= ny) {
-                    continue;
-                }
-                const bool need = mine && abs(oz) <= rzl && abs(oy) <= ryl;
-                if (__ballot(need) == 0ull) continue;
-                const double shy = (double)sy * T.L[1];
-                const bool shifted = (sy != 0) || (sz != 0);
-                const int rowbase = cell_off + nx * (cy + ny * cz);
-                const int ub = start[rowbase + max(cxa - Rx, 0)];
-                const int ue = start[rowbase + min(cxb + Rx, nx - 1) + 1];
-                int jl = 0, jh = 0;
-                if (need) {
-                    jl = start[rowbase + max(kx - rxl, 0)];
-                    jh = start[rowbase + min(kx + rxl, nx - 1) + 1];
-                }
-                for (int pb = ub; pb < ue; pb += kTileP) {
-                    const int pn = min(kTileP, ue - pb);
-                    __syncthreads();
-                    for (int k = lane; k < pn; k += 64) {
-                        const long long g = wo + pb + k;
-                        L.x[k] = B.x[g];
-                        L.y[k] = B.y[g];
-                        L.z[k] = B.z[g];
-                        L.h[k] = B.h[g];
-                        if (KIND == 0) {
-                            L.mw[k] = (R)(B.m[g] * kInvPi);
-                            L.hh[k] = (R)(0.5 * B.h[g]);
-                        } else {
-                            L.m[k] = (R)B.m[g];
-                        }
-                        if (KIND == 1) {
-                            L.vx[k] = (R)B.vx[g];
-                            L.vy[k] = (R)B.vy[g];
-                            L.vz[k] = (R)B.vz[g];
-                            L.rho[k] = (R)B.rho[g];
-                            L.pr2[k] = (R)pr2[pb + k];
-                            L.cs[k] = (R)csav[pb + k];
-                            L.ind[k] = B.ind[g] != 0 ? 1 : 0;
-                        }
-                        if (kStats) L.id[k] = B.id[g];
-                    }
-                    __syncthreads();
-                    const int a = max(jl, pb) - pb;
-                    const int e = min(jh, pb + pn) - pb;
-                    const int len = e > a ? e - a : 0;
-                    const int maxlen = wave_max_int(len);
-                    for (int w0 = 0; w0 < maxlen; w0 += 64, ++slot) {
-                        const int wn = min(64, maxlen - w0);
-                        unsigned long long mask = 0ull;
-                        if (KIND == 1 && use_masks && slot < kMaskSlots) {
-                            mask = mrow[(size_t)slot * 64];
-                        } else if (KIND == 1) {
-                            // no stored mask: each lane tests its own candidates only
-                            const int nt = min(max(len - w0, 0), 64);
-                            for (int t = 0; t < nt; ++t) {
-                                const int u = a + w0 + t;
-                                const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
-                                const uint32_t hit = shifted ? pred1_shift(c, xi, yi, zi, hi, shy, shz)
-                                                             : pred1(c, xi, yi, zi, hi);
-                                mask |= (unsigned long long)hit << t;
-                            }
-                        } else {
-                            // uniform trip count: every lane evaluates wn candidates from a
-                            // base clamped into the padded piece; bits past the lane's own range
-                            // are cleared once per window
-                            const int rem = len - w0;
-                            const unsigned long long valid =
-                                rem >= 64 ? ~0ull : (rem <= 0 ? 0ull : ((1ull << rem) - 1ull));
-                            const int base = min(a + w0, pn);
-                            // candidates are shifted in from the top, so candidate t lands on bit t
-                            const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
-                            // whole groups of 4 (reads past wn stay in the padding; their bits
-                            // are masked)
-                            const int wn4 = (wn + 3) & ~3;
-                            const int n0 = min(wn4, 32);
-                            uint32_t mlo = 0u, mhi = 0u;
-                            auto group = [&](auto shifted_tag, uint32_t m, int t) {
-                                constexpr bool kSh = decltype(shifted_tag)::value;
-#pragma unroll
-                                for (int k = 3; k >= 0; --k) {
-                                    const P4 c{cx_[t + k], cy_[t + k], cz_[t + k], ch_[t + k]};
-                                    m = (m << 1) | (kSh ? pred1_shift(c, xi, yi, zi, hi, shy, shz)
-                                                        : pred1(c, xi, yi, zi, hi));
-                                }
-                                return m;
-                            };
-                            if (!shifted) {
-                                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::false_type{}, mhi, t);
-                                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::false_type{}, mlo, t);
-                            } else {
-                                for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::true_type{}, mhi, t);
-                                for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::true_type{}, mlo, t);
-                            }
-                            mask = (((unsigned long long)mhi << 32) | mlo) & valid;
-                        }
-                        if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
-                        if (KIND == 0) {
-                            // W of each hit; rows without a periodic image skip the shifts
-                            auto hits = [&](auto shifted_tag) {
-                                constexpr bool kSh = decltype(shifted_tag)::value;
-                                while (mask) {
-                                    const int t = __ffsll((long long)mask) - 1;
-                                    mask &= mask - 1ull;
-                                    const int u = a + w0 + t;
-                                    const double dx = L.x[u] - xi;
-                                    const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
-                                    const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
-                                    const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                                    const R r = sqrt_r<R>((R)dd);
-                                    const R ih = rcp<R>(hhi + L.hh[u]);
-                                    a0.add(L.mw[u] * (ih * ih * ih) * spline_w_hit<R>(r * ih));
-                                    if (kStats) {
-                                        ++cnt;
-                                        hash += pair_tag(L.id[u], sy, sz);
-                                    }
-                                }
-                            };
-                            if (shifted) hits(std::true_type{});
-                            else hits(std::false_type{});
-                        }
-                        while (KIND == 1 && mask) {
-                            const int t = __ffsll((long long)mask) - 1;
-                            mask &= mask - 1ull;
-                            const int u = a + w0 + t;
-                            const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
-                            const double dx = c.x - xi;
-                            const double dy = (c.y - yi) + shy;
-                            const double dz = (c.z - zi) + shz;
-                            const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-                            if (dd > 0.0) {
-                                const double hhd = 0.5 * (hi + c.h);
-                                if (sizeof(R) == 4 && L.ind[u] != 0) {
-                                    // a frozen neighbour has no partner term: FP64 keeps the
-                                    // momentum it injects exact to rounding
-                                    const long long g = wo + pb + u;
-                                    double coef, edc;
-                                    force_terms<double>(sqrt(dd), hhd, dx, dy, dz, vxi - B.vx[g], vyi - B.vy[g],
-                                                        vzi - B.vz[g], rhoi_d, B.rho[g], pr2i_d, pr2[pb + u], csi_d,
-                                                        csav[pb + u], mi_d, B.m[g], T.alpha, T.beta, T.eps, plain,
-                                                        coef, edc);
-                                    d0 += dx * coef;
-                                    d1 += dy * coef;
-                                    d2s += dz * coef;
-                                    d3 += edc;
-                                } else {
-                                    R coef, edc;
-                                    const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
-                                    force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, vxi_r - L.vx[u],
-                                                   vyi_r - L.vy[u], vzi_r - L.vz[u], rhoi, L.rho[u], pr2i,
-                                                   L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha, beta, epsv, plain, coef,
-                                                   edc);
-                                    a0.add(fx * coef);
-                                    a1.add(fy * coef);
-                                    a2.add(fz * coef);
-                                    a3.add(edc);
-                                }
-                            }
-                        }
-                    }
-                }
+            int cy = ky0 + oy, sy = 0;
+            if (T.per[1]) {
+                sy = floordiv(cy, ny);
+                cy -= sy * ny;
+            } else if (cy < 0 || cy >= ny) {
+                return g;
             }
+            const bool need = mine && abs(oz) <= rzl && abs(oy) <= ryl;
+            if (__ballot(need) == 0ull) return g;
+            const int rowbase = cell_off + nx * (cy + ny * cz);
+            g.on = true;
+            g.sy = sy;
+            g.sz = sz;
+            g.shy = (double)sy * T.L[1];
+            g.shz = (double)sz * T.L[2];
+            g.shifted = (sy != 0) || (sz != 0);
+            g.ub = uni(start[rowbase + max(cxa - Rx, 0)]);
+            g.ue = uni(start[rowbase + min(cxb + Rx, nx - 1) + 1]);
+            g.jl = 0;
+            g.jh = 0;
+            if (need) {
+                g.jl = start[rowbase + max(kx - rxl, 0)];
+                g.jh = start[rowbase + min(kx + rxl, nx - 1) + 1];
+            }
+            return g;
+        };
+        // rows in mirror pairs (r, nrows-1-r): the hits of a target in a row and in its
+        // mirror row add up to nearly the same count for every lane
+        for (int v = 0; v < (nrows + 1) / 2; ++v) {
+            const RowSeg A = row_seg(v);
+            const int w = nrows - 1 - v;
+            const RowSeg B2 = w != v ? row_seg(w) : RowSeg{};
+            if (A.on && B2.on && run_pair(A, B2)) continue;
+            if (A.on) run_row(A);
+            if (B2.on) run_row(B2);
         }
         pending = pending && !mine;
     }
