@@ -53,7 +53,7 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
 
 
 # kernels behind each pass, as named in the rocprofv3 summaries under profiles/
-def pass_kernels(stage, tile=192):
+def pass_kernels(stage, tile=160):
     """Kernel names (as rocprofv3 reports them) of one pass of the step."""
     return {
         "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
@@ -64,7 +64,7 @@ def pass_kernels(stage, tile=192):
     }.get(stage, [])
 
 
-def measured_traffic(workload, stage, tile=192):
+def measured_traffic(workload, stage, tile=160):
     """HBM bytes per launch of a pass from the committed FETCH_SIZE/WRITE_SIZE summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not os.path.exists(path):
@@ -103,7 +103,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default piece size (192)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default piece size (160)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -177,7 +177,7 @@ def main():
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else 192)
+                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else 160)
                      if args.variant == 1 else None,
                      "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms},

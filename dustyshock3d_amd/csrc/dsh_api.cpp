@@ -94,7 +94,7 @@ int dsh_set_pair_variant(dsh_ctx* c, int variant) {
 }
 
 int dsh_set_tile(dsh_ctx* c, int tile) {
-    if (!c || (tile != 128 && tile != 192 && tile != 256)) return DSH_ERR_ARG;
+    if (!c || !dsh::valid_tile(tile)) return DSH_ERR_ARG;
     c->eng->set_tile(tile);
     return DSH_OK;
 }

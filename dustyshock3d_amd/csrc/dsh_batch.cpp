@@ -115,7 +115,7 @@ int dsh_batch_set_pair_variant(dsh_batch* b, int variant) {
 }
 
 int dsh_batch_set_tile(dsh_batch* b, int tile) {
-    if (!b || (tile != 128 && tile != 192 && tile != 256)) return DSH_ERR_ARG;
+    if (!b || !dsh::valid_tile(tile)) return DSH_ERR_ARG;
     b->eng->set_tile(tile);
     return DSH_OK;
 }

@@ -579,7 +579,8 @@ int Engine::autotune(int steps_per_trial, bool with_variants, int* tile_out, int
     struct Cand {
         int tile, variant;
     };
-    std::vector<Cand> cands = {{128, kPairTiled}, {192, kPairTiled}, {256, kPairTiled}};
+    std::vector<Cand> cands;
+    for (int t : kTiles) cands.push_back({t, kPairTiled});
     if (with_variants) cands.push_back({192, kPairDirect});
     auto finished = [&]() {
         for (const TubeDyn& d : dyn_h_)

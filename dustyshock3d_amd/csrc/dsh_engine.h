@@ -65,8 +65,8 @@ public:
         graph_dirty_ = true;
     }
     int pair_variant() const { return pair_variant_; }
-    // candidates staged per piece in the tiled pair kernels (128, 192 or 256); every size
-    // gives the same results bit for bit
+    // candidates staged per piece in the tiled pair kernels (kTiles); every size gives the
+    // same results bit for bit
     void set_tile(int t) {
         tile_ = t;
         a_.tile = t;
@@ -109,7 +109,7 @@ private:
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
     int precision_ = 0;
     int pair_variant_ = kPairTiled;
-    int tile_ = 192;
+    int tile_ = 160;
     long long ntot_ = 0;
     size_t dev_bytes_ = 0;
     std::vector<std::pair<std::string, double>> ktimes_;

@@ -20,6 +20,13 @@ struct Map {
 // wave per 64-target chunk with LDS-staged candidate rows
 constexpr int kPairDirect = 0;
 constexpr int kPairTiled = 1;
+// Candidates staged per LDS piece by the tiled pair kernels (template instances).
+constexpr int kTiles[] = {128, 160, 192, 224, 256};
+inline bool valid_tile(int t) {
+    for (int k : kTiles)
+        if (k == t) return true;
+    return false;
+}
 
 struct StepArgs {
     const TubeConst* tc;
@@ -48,7 +55,7 @@ struct StepArgs {
     unsigned long long* masks;   // gas density hit masks reused by the force pass
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
     int variant;
-    int tile;              // candidates staged per piece in the tiled pair kernels: 128, 192, 256
+    int tile;              // candidates staged per piece in the tiled pair kernels (kTiles)
 };
 
 // scratch: 5 doubles per particle block of pmap

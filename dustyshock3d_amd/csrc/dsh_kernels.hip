@@ -793,7 +793,7 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
 // 64 candidates, and does the pair arithmetic for the hits alone.  The candidate set of
 // a lane is the one of the direct kernels (cells within ceil(reach / c_a) of its own),
 // so the neighbour sets are identical.
-// candidates staged per piece: a template parameter (128, 192 or 256) chosen at run time;
+// candidates staged per piece: a template parameter (128 to 256, kTiles) chosen at run time;
 // the pieces split each row's range in order, so every size gives the same sums bit for bit
 constexpr int kMaskSlots = 48;  // stored hit-mask words per lane of a gas chunk
 
@@ -890,8 +890,11 @@ __device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dv
 // The gas density pass stores its predicate hit masks (one 64-bit word per lane per
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
 // the same iteration order and reads them instead of re-running the predicate.
+// Occupancy targets: both passes are latency- as well as VALU-bound, so the register
+// budget is capped at 4 (force) / 5 (density) waves per SIMD; with pieces of 160 candidates
+// the LDS of a wave (10 KB force, 8.4 KB density) allows as many.
 template <typename R, int KIND, bool kStats, int kTileP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                                     PBuf B, const int* __restrict__ start,
                                                     const int* __restrict__ wkey, const double* __restrict__ reach,
@@ -1070,11 +1073,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             if (sizeof(R) == 4 && L.ind[u] != 0) {
                 // a frozen neighbour has no partner term: FP64 keeps the momentum it
                 // injects exact to rounding
-                // (the target's FP64 values are read again here rather than kept in registers)
-                const long long g = wo + gi, gq = wo + q;
+                // (the target's FP64 values are read again here rather than kept in registers;
+                // the empty asm keeps their addresses from being hoisted out of the loops)
+                int qq = q;
+                asm volatile("" : "+v"(qq));
+                const long long g = wo + gi, gq = wo + qq;
                 double coef, edc;
                 force_terms<double>(sqrt(dd), hhd, dx, dy, dz, B.vx[gq] - B.vx[g], B.vy[gq] - B.vy[g],
-                                    B.vz[gq] - B.vz[g], B.rho[gq], B.rho[g], pr2[q], pr2[gi], csav[q], csav[gi],
+                                    B.vz[gq] - B.vz[g], B.rho[gq], B.rho[g], pr2[qq], pr2[gi], csav[qq], csav[gi],
                                     B.m[gq], B.m[g], T.alpha, T.beta, T.eps, plain, coef, edc);
                 d0 += dx * coef;
                 d1 += dy * coef;
@@ -1684,8 +1690,10 @@ template <typename R, int KIND, bool kStats>
 static void launch_tiled(const StepArgs& a, const Map& m, hipStream_t s) {
     switch (a.tile) {
         case 128: launch_tiled_t<R, KIND, kStats, 128>(a, m, s); break;
+        case 192: launch_tiled_t<R, KIND, kStats, 192>(a, m, s); break;
+        case 224: launch_tiled_t<R, KIND, kStats, 224>(a, m, s); break;
         case 256: launch_tiled_t<R, KIND, kStats, 256>(a, m, s); break;
-        default: launch_tiled_t<R, KIND, kStats, 192>(a, m, s); break;
+        default: launch_tiled_t<R, KIND, kStats, 160>(a, m, s); break;
     }
 }
 

@@ -156,7 +156,7 @@ class Simulation:
                 "n_avg": int(i[4])}
 
     def set_tile(self, tile: int):
-        """Candidates staged per LDS piece by the tiled pair kernels (128, 192, 256)."""
+        """Candidates staged per LDS piece by the tiled pair kernels (128, 160, 192, 224, 256)."""
         self._call(self._lib.dsh_set_tile(self._ctx, int(tile)))
 
     def autotune(self, steps_per_trial: int = 2, with_variants: bool = False) -> dict:

@@ -9,6 +9,7 @@ from dustyshock3d_amd.batch import Batch
 
 pytestmark = pytest.mark.gpu
 
+TILES = (160, 128, 192, 224, 256)
 FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "h", "e", "t_stop", "ax", "ay", "az", "edot", "nn", "cell")
 
 
@@ -16,15 +17,15 @@ FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "h", "e", "t_stop", "ax", "ay"
 def test_piece_sizes_agree_bit_for_bit(precision):
     spec = configs.cfg0(precision=precision, max_steps=20)
     runs = {}
-    for tile in (128, 192, 256):
+    for tile in TILES:
         s = Simulation.from_spec(spec)
         s.set_tile(tile)
         s.step(20)
         runs[tile] = [s.download(sp) for sp in (0, 1)]
-    for tile in (128, 256):
+    for tile in TILES[1:]:
         for sp in (0, 1):
             for k in FIELDS:
-                assert np.array_equal(runs[tile][sp][k], runs[192][sp][k]), (tile, sp, k)
+                assert np.array_equal(runs[tile][sp][k], runs[TILES[0]][sp][k]), (tile, sp, k)
 
 
 def test_autotune_keeps_the_answer():
@@ -32,7 +33,7 @@ def test_autotune_keeps_the_answer():
     b = Batch(specs)
     res = b.autotune(steps_per_trial=2)
     print("\n", res)
-    assert res["tile"] in (128, 192, 256) and res["variant"] == abi.PAIR_TILED and res["ms_per_step"] > 0
+    assert res["tile"] in TILES and res["variant"] == abi.PAIR_TILED and res["ms_per_step"] > 0
     b.run(100)
     ref = Batch(specs)
     ref.run(100)
