@@ -323,6 +323,7 @@ int Engine::prepare() {
     int** ifields[] = {&a_.key, &a_.slot, &a_.tsrc, &a_.tid, &a_.tkey, &a_.wkey, &a_.ncount};
     for (int** f : ifields)
         if ((rc = alloc(f, (size_t)ntot_))) return rc;
+    if ((rc = alloc(&a_.tx, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.pr2, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.csav, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.nhash, (size_t)ntot_))) return rc;
@@ -705,7 +706,21 @@ int Engine::download(int tube, int species, const dsh_soa_view* v) {
         for (size_t q = 0; q < n; ++q) {
             const int id = idd[q];
             if (v->cell) v->cell[id] = stepped ? wk[q] - C.cell_off[species] : -1;
-            if (v->nn) v->nn[id] = (stepped && q + 1 < n && wk[q + 1] == wk[q]) ? idd[q + 1] : -1;
+            if (v->nn) v->nn[id] = -1;
+        }
+        if (v->nn && stepped) {
+            // the reference lists run through a cell's members in index order (kernel.cu:1937-1995);
+            // the device keeps a cell's members in (x, id) order, so each run of equal keys is
+            // re-ordered by id here
+            std::vector<int> members;
+            for (size_t q0 = 0; q0 < n;) {
+                size_t q1 = q0 + 1;
+                while (q1 < n && wk[q1] == wk[q0]) ++q1;
+                members.assign(idd.begin() + q0, idd.begin() + q1);
+                std::sort(members.begin(), members.end());
+                for (size_t k = 0; k + 1 < members.size(); ++k) v->nn[members[k]] = members[k + 1];
+                q0 = q1;
+            }
         }
     }
     return DSH_OK;

@@ -36,6 +36,7 @@ struct StepArgs {
     int ntubes;
     // per particle (global sorted space)
     int *key, *slot, *tsrc, *tid, *tkey, *wkey;
+    unsigned long long* tx;  // x in scatter order as ordered bits (the sort's secondary key)
     double *pr2, *csav;
     int* ncount;
     unsigned long long* nhash;

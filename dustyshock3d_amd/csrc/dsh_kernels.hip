@@ -435,10 +435,18 @@ __global__ __launch_bounds__(kBS) void k_scan_down(const int* __restrict__ in, i
     if (base + kScanItems >= n && base < n) out[n] = run;  // the thread holding the last item
 }
 
+// The bits of a double as an unsigned integer that orders like the value (NaNs included,
+// so an order exists even for a failed tube).
+__device__ __forceinline__ unsigned long long order_bits(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+
 __global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__ cur, PBuf B,
                                                  const int* __restrict__ key, const int* __restrict__ slot,
                                                  const int* __restrict__ start, int* __restrict__ tsrc,
-                                                 int* __restrict__ tid, int* __restrict__ tkey) {
+                                                 int* __restrict__ tid, int* __restrict__ tkey,
+                                                 unsigned long long* __restrict__ tx) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
@@ -449,25 +457,55 @@ __global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__
     tsrc[pos] = p;
     tid[pos] = B.id[so + p];
     tkey[pos] = k;
+    tx[pos] = order_bits(B.x[so + p]);
 }
 
-// Final position = cell start + number of cell members with a smaller id.
+
+constexpr int kRankRun = 1024;   // cell members staged per block by k_rank_permute
+
+// Final position = cell start + number of cell members before it in (x, id) order: cells
+// are sorted by x, so a row of cells is one x-sorted run for the pair kernels' windows.
 // A finished tube is still sorted every step (its positions moved on its last drift, so
 // the order can change once); its last acceleration and de/dt move with it.
 __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __restrict__ td,
                                                       const int* __restrict__ cur, PBuf B,
                                                       const int* __restrict__ start, const int* __restrict__ tsrc,
                                                       const int* __restrict__ tid, const int* __restrict__ tkey,
-                                                      int* __restrict__ wkey, int* __restrict__ ginv) {
+                                                      const unsigned long long* __restrict__ tx,
+                                                      int* __restrict__ wkey,
+                                                      int* __restrict__ ginv) {
+    // the block's positions lie in one segment and keys ascend along them, so the members of
+    // every cell the block touches form one run [w0, w1), staged in LDS when it fits
+    __shared__ unsigned long long sx[kRankRun];
+    __shared__ int sid[kRankRun];
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
+    const int qf = ip.sg.off + ip.li;
+    const int ql = ip.sg.off + min(ip.li + kBS, ip.sg.n) - 1;
+    const int w0 = start[tkey[qf]], w1 = start[tkey[ql] + 1];
+    const bool staged = w1 - w0 <= kRankRun;
+    if (staged) {
+        for (int k = threadIdx.x; k < w1 - w0; k += kBS) {
+            sx[k] = tx[w0 + k];
+            sid[k] = tid[w0 + k];
+        }
+    }
+    __syncthreads();
     if (li >= ip.sg.n) return;
     const int q = ip.sg.off + li;
     const int c = tkey[q];
     const int lo = start[c], hi = start[c + 1];
     const int my = tid[q];
+    const unsigned long long mx = tx[q];
     int r = 0;
-    for (int k = lo; k < hi; ++k) r += (tid[k] < my) ? 1 : 0;
+    if (staged) {
+        for (int k = lo - w0; k < hi - w0; ++k) r += (sx[k] < mx || (sx[k] == mx && sid[k] < my)) ? 1 : 0;
+    } else {
+        for (int k = lo; k < hi; ++k) {
+            const unsigned long long kx = tx[k];
+            r += (kx < mx || (kx == mx && tid[k] < my)) ? 1 : 0;
+        }
+    }
     const int dst = lo + r;
     const int src = tsrc[q];
     const long long so = (long long)(*cur) * B.ntot;
@@ -928,6 +966,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     // hit masks: written by the gas density pass, read by the force pass
     const bool use_masks = masks != nullptr && s == 0 && b < mask_chunks;
     unsigned long long* mrow = use_masks ? masks + (size_t)b * kMaskSlots * 64 + lane : nullptr;
+    // after the masks: each lane's x-window (start << 16 | length) per visited row segment,
+    // written by the gas density pass, read by the force pass in place of its own search
+    unsigned int* wrow =
+        use_masks ? reinterpret_cast<unsigned int*>(masks + (size_t)mask_chunks * kMaskSlots * 64) +
+                        (size_t)b * kMaskSlots * 64 + lane
+                  : nullptr;
+    int segi = 0;
     int slot = 0;
 
     Acc<R> a0, a1, a2, a3;  // density: a0; force: ax, ay, az, de/dt
@@ -1110,8 +1155,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     };
     // Hits of two windows: a lane takes all of its hits of the first (candidates from sorted
     // index gA + staged slot), then of the second (gB + slot), so the loop runs max over
-    // lanes of the total, not the sum of the two maxima.  The order per lane is the one of
-    // hits1(first); hits1(second).
+    // lanes of the total, not the sum of the two maxima.  Per lane: the first window's hits
+    // in candidate order, then the second's.
     auto hits2 = [&](auto shifted_tag, unsigned long long mA, int oA, int gA, const RowSeg& A,
                      unsigned long long mB, int oB, int gB, const RowSeg& B2) {
         bool second = mA == 0ull;
@@ -1130,48 +1175,123 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             }
         }
     };
-    // All pieces and windows of one row, in order.
-    auto run_row = [&](const RowSeg& g) {
-        for (int pb = g.ub; pb < g.ue; pb += kTileP) {
-            const int pn = min(kTileP, g.ue - pb);
-            stage(pb, pn, 0, 0);
-            const int a = max(g.jl, pb) - pb;
-            const int e = min(g.jh, pb + pn) - pb;
-            const int len = e > a ? e - a : 0;
-            const int maxlen = uni(wave_max_int(len));
-            for (int w0 = 0; w0 < maxlen; w0 += 64) {
-                const unsigned long long mask = window_mask(a + w0, len - w0, maxlen - w0, pn, g);
-                if (KIND == 1) {
-                    // one copy of the large force body: the image shift is added always
-                    // (+0.0 for a row without one, which leaves the difference unchanged)
-                    hits2(std::true_type{}, mask, a + w0, pb, g, 0ull, 0, pb, g);
-                } else if (g.shifted) {
-                    hits1(std::true_type{}, mask, a + w0, pb, g);
+    // This lane's x-window inside staged slots [a, e) of one row: rows are sorted by x (cells
+    // ascend in x, a cell by x then id), so the candidates with -rch < x_j - x_i < rch (the
+    // differences rounded as the predicate rounds them) are one run.  Any other candidate has
+    // d2 >= dx^2 >= rch^2 > (h_i + h_j)^2 and cannot be a neighbour.  Returns the run as
+    // (start, length); binary searches on the staged x.
+    auto xwindow = [&](int a, int e, int& len) {
+        if (e <= a) {
+            len = 0;
+            return a;
+        }
+        int lo = a, n = e - a;
+        while (n > 0) {
+            const int hh = n >> 1;
+            if (L.x[lo + hh] - xi <= -rch) {
+                lo += hh + 1;
+                n -= hh + 1;
+            } else {
+                n = hh;
+            }
+        }
+        int hi = lo;
+        n = e - lo;
+        while (n > 0) {
+            const int hh = n >> 1;
+            if (L.x[hi + hh] - xi < rch) {
+                hi += hh + 1;
+                n -= hh + 1;
+            } else {
+                n = hh;
+            }
+        }
+        len = hi - lo;
+        return lo;
+    };
+    // The x-window of this lane in one row segment of the current piece (staged slots
+    // [a0, e0)): searched by the density pass and stored for gas chunks, read back by the
+    // force pass.
+    auto seg_window_range = [&](int a0, int e0, int& len) {
+        int a;
+        if (KIND == 1 && use_masks && segi < kMaskSlots) {
+            const unsigned int w = wrow[(size_t)segi * 64];
+            a = (int)(w >> 16);
+            len = (int)(w & 0xffffu);
+        } else {
+            a = xwindow(a0, e0, len);
+            if (KIND == 0 && use_masks && segi < kMaskSlots) wrow[(size_t)segi * 64] = ((unsigned)a << 16) | (unsigned)len;
+        }
+        ++segi;
+        return a;
+    };
+    // One visit: row A alone, in pieces of kTileP, or rows A and B2 staged together as one
+    // piece (pair; only when both fit).  Per piece each row is a segment of staged slots; a
+    // lane's x-window in a segment is cut into windows of 64 candidates, and window masks
+    // take slots in the order A's windows, then B2's.  When each segment has at most one
+    // window, one hit loop takes both (a lane runs its A hits, then its B2 hits); otherwise
+    // one window per iteration.  Either way a target sums its hits in the order A then B2,
+    // candidates in sorted order, so sums do not depend on the piece size or on whether a
+    // pair fitted.  The pair kernels' large bodies (predicate windows, force hits) each
+    // have one call site here.
+    auto visit = [&](const RowSeg& A, const RowSeg& B2, bool pair) {
+        const int nA = A.ue - A.ub, nB = pair ? B2.ue - B2.ub : 0;
+        // a pair is one piece even when A's range is empty
+        const int npieces = pair ? 1 : (nA + kTileP - 1) / kTileP;
+        for (int ip = 0; ip < npieces; ++ip) {
+            const int pb = A.ub + ip * kTileP;
+            const int pnA = pair ? nA : min(kTileP, A.ue - pb);
+            const int pn = pnA + nB;
+            stage(pb, pnA, B2.ub, nB);
+            int lenA, lenB = 0;
+            const int aA = seg_window_range(max(A.jl, pb) - pb, min(A.jh, pb + pnA) - pb, lenA);
+            const int aB =
+                pair ? seg_window_range(nA + max(B2.jl, B2.ub) - B2.ub, nA + min(B2.jh, B2.ue) - B2.ub, lenB) : 0;
+            const int mlA = uni(wave_max_int(lenA));
+            const int mlB = pair ? uni(wave_max_int(lenB)) : 0;
+            const int winA = (mlA + 63) >> 6, winB = (mlB + 63) >> 6;
+            const bool both = winA <= 1 && winB <= 1;
+            const int nit = both ? 1 : winA + winB;
+            for (int k = 0; k < nit; ++k) {
+                unsigned long long mA = 0ull, mB = 0ull;
+                int oA = aA, oB = aB;
+                // the window of segment `second` taken at iteration k, if any
+                auto seg_window = [&](bool second) {
+                    const int w = both ? 0 : (second ? k - winA : k);
+                    const bool use = both ? (second ? winB : winA) > 0 : (second ? k >= winA : k < winA);
+                    if (!use) return;
+                    const int o = (second ? aB : aA) + 64 * w;
+                    const unsigned long long m =
+                        window_mask(o, (second ? lenB : lenA) - 64 * w, (second ? mlB : mlA) - 64 * w, pn,
+                                    second ? B2 : A);
+                    if (second) {
+                        mB = m;
+                        oB = o;
+                    } else {
+                        mA = m;
+                        oA = o;
+                    }
+                };
+                if (KIND == 0) {
+                    // small bodies: specialised copies
+                    seg_window(false);
+                    if (pair) seg_window(true);
+                    if (!pair) {
+                        if (A.shifted) hits1(std::true_type{}, mA, oA, pb, A);
+                        else hits1(std::false_type{}, mA, oA, pb, A);
+                    } else if (A.shifted || B2.shifted) {
+                        hits2(std::true_type{}, mA, oA, pb, A, mB, oB, B2.ub - nA, B2);
+                    } else {
+                        hits2(std::false_type{}, mA, oA, pb, A, mB, oB, B2.ub - nA, B2);
+                    }
                 } else {
-                    hits1(std::false_type{}, mask, a + w0, pb, g);
+                    // one copy of the large bodies
+#pragma clang loop unroll(disable)
+                    for (int sg2 = 0; sg2 < 2; ++sg2) seg_window(sg2 == 1);
+                    hits2(std::true_type{}, mA, oA, pb, A, mB, oB, B2.ub - nA, B2);
                 }
             }
         }
-    };
-    // Two rows staged as one piece when both fit and every lane's range in each is one
-    // window; same windows, slots and per-lane order as run_row(A); run_row(B).
-    auto run_pair = [&](const RowSeg& A, const RowSeg& B2) -> bool {
-        const int nA = A.ue - A.ub, nB = B2.ue - B2.ub;
-        if (nA + nB > kTileP) return false;
-        const int lenA = A.jh > A.jl ? A.jh - A.jl : 0;
-        const int lenB = B2.jh > B2.jl ? B2.jh - B2.jl : 0;
-        const int mlA = uni(wave_max_int(lenA)), mlB = uni(wave_max_int(lenB));
-        if (mlA > 64 || mlB > 64) return false;
-        stage(A.ub, nA, B2.ub, nB);
-        const int aA = max(A.jl, A.ub) - A.ub;
-        const int aB = nA + max(B2.jl, B2.ub) - B2.ub;
-        const unsigned long long mA = mlA > 0 ? window_mask(aA, lenA, mlA, nA + nB, A) : 0ull;
-        const unsigned long long mB = mlB > 0 ? window_mask(aB, lenB, mlB, nA + nB, B2) : 0ull;
-        if (KIND == 1 || A.shifted || B2.shifted)
-            hits2(std::true_type{}, mA, aA, A.ub, A, mB, aB, B2.ub - nA, B2);
-        else
-            hits2(std::false_type{}, mA, aA, A.ub, A, mB, aB, B2.ub - nA, B2);
-        return true;
     };
 
     bool pending = active;
@@ -1232,9 +1352,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             const RowSeg A = row_seg(v);
             const int w = nrows - 1 - v;
             const RowSeg B2 = w != v ? row_seg(w) : RowSeg{};
-            if (A.on && B2.on && run_pair(A, B2)) continue;
-            if (A.on) run_row(A);
-            if (B2.on) run_row(B2);
+            // one call site of visit: the pair, or A and then B2 alone
+            const bool pair = A.on && B2.on && (A.ue - A.ub) + (B2.ue - B2.ub) <= kTileP;
+#pragma clang loop unroll(disable)
+            for (int k = 0; k < (pair ? 1 : 2); ++k) {
+                const bool first = k == 0;
+                if (!(first ? A.on : B2.on)) continue;
+                visit(first ? A : B2, first ? B2 : A, pair);
+            }
         }
         pending = pending && !mine;
     }
@@ -1627,7 +1752,8 @@ inline dim3 grid1(int n) { return dim3((unsigned)((n + kBS - 1) / kBS)); }
 
 int scan_blocks(int n) { return (n + kScanTile - 1) / kScanTile; }
 
-size_t mask_words(int gas_chunks) { return (size_t)gas_chunks * kMaskSlots * 64; }
+// hit masks (one 64-bit word per lane and slot) followed by the x-windows (32 bits each)
+size_t mask_words(int gas_chunks) { return (size_t)gas_chunks * kMaskSlots * 64 * 3 / 2; }
 
 hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s) {
     hipLaunchKernelGGL(k_prepare_part, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.B, scratch);
@@ -1656,13 +1782,13 @@ hipError_t launch_scan(const StepArgs& a, hipStream_t s) {
 
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.key, a.slot,
-                       a.start, a.tsrc, a.tid, a.tkey);
+                       a.start, a.tsrc, a.tid, a.tkey, a.tx);
     return hipGetLastError();
 }
 
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.cur, a.B, a.start,
-                       a.tsrc, a.tid, a.tkey, a.wkey, a.ginv);
+                       a.tsrc, a.tid, a.tkey, a.tx, a.wkey, a.ginv);
     return hipGetLastError();
 }
 
