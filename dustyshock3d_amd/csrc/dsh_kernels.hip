@@ -102,7 +102,7 @@ template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return 
 // v_sqrt_f32 (1 ulp); arguments are squared distances, never denormal-small in practice.
 template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Plain sum in FP64, Kahan-compensated sum in FP32.
+// Plain sum in FP64; FP32 terms are summed in FP64 too (one convert and one add per term).
 template <typename R> struct Acc;
 template <> struct Acc<double> {
     double s = 0.0;
@@ -110,14 +110,9 @@ template <> struct Acc<double> {
     __device__ __forceinline__ double get() const { return s; }
 };
 template <> struct Acc<float> {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float v) {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
+    double s = 0.0;
+    __device__ __forceinline__ void add(float v) { s += (double)v; }
+    __device__ __forceinline__ double get() const { return s; }
 };
 
 // Cubic spline W(r,h) (kernel.cu:11-22) and dW/dr for r >= 0 (kernel.cu:24-38),
@@ -902,21 +897,33 @@ struct TileLds {
 
 // Pressure + viscosity terms of one pair (IDIC.h:225-250): returns the acceleration
 // coefficient of (dx,dy,dz) and the de/dt contribution.
+// d2 is the squared distance: FP64 takes sqrt and 1/r; FP32 takes 1/r from v_rsq_f32 and
+// r = d2/r, one transcendental instead of two.
 template <typename T>
-__device__ __forceinline__ void force_terms(T dist, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
+__device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
                                             T pr2i, T pr2j, T csi, T csj, T mi, T mj, T alpha, T beta, T epsv,
                                             bool plain, T& coef, T& edc) {
+    T dist, rinv, dist2;
+    if constexpr (sizeof(T) == 8) {
+        dist = sqrt(d2);
+        rinv = rcp<T>(dist);
+        dist2 = dist * dist;
+    } else {
+        rinv = __builtin_amdgcn_rsqf(d2);
+        dist = d2 * rinv;
+        dist2 = d2;
+    }
     const T ih = rcp<T>(hh);
     // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
     const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
     T fnu = T(0);
     if (nu < T(0)) {
         const T cnu = T(0.5) * (csi + csj);
-        const T mu = hh * nu * rcp<T>(dist * dist + epsv * hh * hh);
+        const T mu = hh * nu * rcp<T>(dist2 + epsv * hh * hh);
         fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
     }
     const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw_hit<T>(dist * ih);
-    const T dwr = dw * rcp<T>(dist);
+    const T dwr = dw * rinv;
     if (plain)
         edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
     else
@@ -1124,7 +1131,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 asm volatile("" : "+v"(qq));
                 const long long g = wo + gi, gq = wo + qq;
                 double coef, edc;
-                force_terms<double>(sqrt(dd), hhd, dx, dy, dz, B.vx[gq] - B.vx[g], B.vy[gq] - B.vy[g],
+                force_terms<double>(dd, hhd, dx, dy, dz, B.vx[gq] - B.vx[g], B.vy[gq] - B.vy[g],
                                     B.vz[gq] - B.vz[g], B.rho[gq], B.rho[g], pr2[qq], pr2[gi], csav[qq], csav[gi],
                                     B.m[gq], B.m[g], T.alpha, T.beta, T.eps, plain, coef, edc);
                 d0 += dx * coef;
@@ -1134,7 +1141,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             } else {
                 R coef, edc;
                 const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
-                force_terms<R>(sqrt_r<R>((R)dd), (R)hhd, fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
+                force_terms<R>((R)dd, (R)hhd, fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
                                vzi_r - L.vz[u], rhoi, L.rho[u], pr2i, L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha,
                                beta, epsv, plain, coef, edc);
                 a0.add(fx * coef);
