@@ -838,15 +838,16 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-__device__ __forceinline__ int wave_min_int(int v) {
+// Wave maximum of v in [0, 512) by a bit-by-bit search on ballots: comparisons and scalar
+// operations only, no cross-lane data movement through LDS.
+__device__ __forceinline__ int wave_max_u9(int v) {
+    int r = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_int(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
+    for (int b = 8; b >= 0; --b) {
+        const int c = r | (1 << b);
+        if (__ballot(v >= c) != 0ull) r = c;
+    }
+    return r;
 }
 // a value equal on every lane, as a scalar (kept in SGPRs)
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -946,6 +947,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                                                     double* __restrict__ pr2, double* __restrict__ csav,
                                                     int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
                                                     unsigned long long* __restrict__ masks, int mask_chunks) {
+    static_assert(kTileP <= 256, "window lengths are reduced with wave_max_u9");
     __shared__ TileLds<R, KIND, kStats, kTileP> L;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int sgi = find_seg(mp, b);
@@ -1254,8 +1256,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             const int aA = seg_window_range(max(A.jl, pb) - pb, min(A.jh, pb + pnA) - pb, lenA);
             const int aB =
                 pair ? seg_window_range(nA + max(B2.jl, B2.ub) - B2.ub, nA + min(B2.jh, B2.ue) - B2.ub, lenB) : 0;
-            const int mlA = uni(wave_max_int(lenA));
-            const int mlB = pair ? uni(wave_max_int(lenB)) : 0;
+            // lengths are within one piece (<= 256 candidates)
+            const int mlA = wave_max_u9(lenA);
+            const int mlB = pair ? wave_max_u9(lenB) : 0;
             const int winA = (mlA + 63) >> 6, winB = (mlB + 63) >> 6;
             const bool both = winA <= 1 && winB <= 1;
             const int nit = both ? 1 : winA + winB;
@@ -1308,8 +1311,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
         const int leader = __ffsll((long long)pm) - 1;
         const int row0 = uni(__shfl(row, leader, 64));
         const bool mine = pending && row == row0;
-        const int cxa = uni(wave_min_int(mine ? kx : 0x7fffffff));
-        const int cxb = uni(wave_max_int(mine ? kx : -1));
+        // targets are sorted by cell, so along the lanes of one x-row kx does not decrease
+        const unsigned long long mm = __ballot(mine);
+        const int cxa = __builtin_amdgcn_readlane(kx, __ffsll((long long)mm) - 1);
+        const int cxb = __builtin_amdgcn_readlane(kx, 63 - __builtin_clzll(mm));
         const double rmax = wave_max(mine ? rch : 0.0);
         const int Rx = uni((int)ceil(rmax * T.ic[0]));
         const int Ry = uni((int)ceil(rmax * T.ic[1]));
