@@ -3,7 +3,7 @@
 
 Workload (cfg2): isothermal dusty shock tube, x in [-0.5,0.5], y,z in [0,0.2) periodic,
 lattice spacing 0.002 / 0.004, density 1 / 0.125, dust-to-gas 1, K = 100, seeded jitter,
-CFL 0.3 step, mixed precision (FP32 pair terms summed in FP64; FP64 drag,
+CFL 0.3 step, mixed precision (FP32 pair sums with Kahan compensation; FP64 drag,
 positions and conservation sums).  Inputs are resident in HBM before the timed region.
 
 A "step" is one pass of the whole per-step pipeline (sort, density, force, IDIC drag +
@@ -167,7 +167,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair terms (f64 sums) + f64 drag/positions/reductions" if spec.precision == 1 else "f64",
+        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions" if spec.precision == 1 else "f64",
         "data": "synthetic (seeded lattice shock tube)",
         "config": {"workload": f"{args.config}: dusty shock tube, {n_gas} gas + {n_dust} dust particles, "
                                f"y,z width {spec.y_width}, dx {spec.dx_left}/{2 * spec.dx_left}, K={spec.K}, CFL 0.3",
@@ -246,7 +246,7 @@ def sweep_mode(args):
         "wall_s": wall, "setup_s": setup, "particle_steps": ps,
         "tubes": len(specs), "steps_min_max": [min(steps), max(steps)],
         "time_to_solution_s": {"p50": float(np.percentile(tts, 50)), "p99": float(np.percentile(tts, 99))},
-        "dtype": "f32 pair terms (f64 sums) + f64 drag/positions/reductions", "data": "synthetic",
+        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
         "config": {"workload": "cfg3 sweep: y,z in [0,0.05], dx log-uniform [0.0025,0.01], K log-uniform "
                                "[0.01,1000], dust-to-gas uniform [0.01,10], each to t=0.2",
                    "pair_kernels": "tiled" if args.variant == 1 else "direct"},
@@ -320,7 +320,7 @@ def serve_mode(args):
                                            "p99": float(np.percentile(serial_tts, 99))},
         "serial_particle_steps_per_s": float(psteps.sum() / service.sum()),
         "wall_s": wall, "rebuilds": st["rebuilds"], "epochs": st["epochs"], "hbm_budget": args.budget,
-        "dtype": "f32 pair terms (f64 sums) + f64 drag/positions/reductions", "data": "synthetic",
+        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
         "config": {"workload": "cfg4: Poisson arrivals, 95% cfg3-distribution tubes, 5% large tubes "
                                "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch},
     }

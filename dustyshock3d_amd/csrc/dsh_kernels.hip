@@ -102,7 +102,7 @@ template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return 
 // v_sqrt_f32 (1 ulp); arguments are squared distances, never denormal-small in practice.
 template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Plain sum in FP64; FP32 terms are summed in FP64 too (one convert and one add per term).
+// Plain sum in FP64, Kahan-compensated sum in FP32.
 template <typename R> struct Acc;
 template <> struct Acc<double> {
     double s = 0.0;
@@ -110,9 +110,14 @@ template <> struct Acc<double> {
     __device__ __forceinline__ double get() const { return s; }
 };
 template <> struct Acc<float> {
-    double s = 0.0;
-    __device__ __forceinline__ void add(float v) { s += (double)v; }
-    __device__ __forceinline__ double get() const { return s; }
+    float s = 0.f, c = 0.f;
+    __device__ __forceinline__ void add(float v) {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
 };
 
 // Cubic spline W(r,h) (kernel.cu:11-22) and dW/dr for r >= 0 (kernel.cu:24-38),

@@ -49,8 +49,8 @@ extern "C" {
 
 /* precision */
 #define DSH_FP64 0           /* all arithmetic in FP64 */
-#define DSH_MIXED 1          /* pair terms in FP32 (summed in FP64), neighbour test, drag,
-                                positions, conservation sums in FP64 */
+#define DSH_MIXED 1          /* pair sums in FP32 with Kahan compensation; neighbour test,
+                                drag, positions, conservation sums in FP64 */
 
 /*
  * Run parameters.  The first block is Init.txt lines 2-18 in file order
