@@ -36,7 +36,7 @@ ALGO_BYTES = {
     "idic": (168, 148),
 }
 AVG_CELL_BYTES = 352
-SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "cellmax", "radius")
+SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "radius")
 
 
 def stage_bytes(stage, n_gas, n_dust, n_avg):
@@ -60,7 +60,7 @@ def pass_kernels(stage, tile=160):
         "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
         "idic": ["k_idic"],
         "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
-                 "k_cellmax", "k_window_max", "k_radius"],
+                 "k_window_max", "k_radius"],
     }.get(stage, [])
 
 

@@ -444,8 +444,9 @@ int Engine::prepare() {
 }
 
 namespace {
-const char* const kStageNames[] = {"begin",   "key",     "scan",    "scatter", "rank_permute", "cellmax",
-                                   "radius",  "density", "force",   "idic",    "finalize",     "flip"};
+// rank_permute also takes each cell's largest h
+const char* const kStageNames[] = {"begin",   "key",   "scan", "scatter",  "rank_permute", "radius",
+                                   "density", "force", "idic", "finalize", "flip"};
 constexpr int kStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 }  // namespace
 
@@ -470,13 +471,12 @@ int Engine::enqueue_step(bool timed) {
             case 2: e = launch_scan(a_, stream_); break;
             case 3: e = launch_scatter(a_, stream_); break;
             case 4: e = launch_rank_permute(a_, stream_); break;
-            case 5: e = launch_cellmax(a_, stream_); break;
-            case 6: e = launch_radius(a_, stream_); break;
-            case 7: e = launch_density(a_, precision_, stats_, stream_); break;
-            case 8: e = launch_force(a_, precision_, stream_); break;
-            case 9: e = launch_idic(a_, stream_); break;
-            case 10: e = launch_finalize(a_, stream_); break;
-            case 11: e = launch_flip(a_, stream_); break;
+            case 5: e = launch_radius(a_, stream_); break;
+            case 6: e = launch_density(a_, precision_, stats_, stream_); break;
+            case 7: e = launch_force(a_, precision_, stream_); break;
+            case 8: e = launch_idic(a_, stream_); break;
+            case 9: e = launch_finalize(a_, stream_); break;
+            case 10: e = launch_flip(a_, stream_); break;
         }
         if (e != hipSuccess) return hip_check(e, kStageNames[i]);
         if (timed) (void)hipEventRecord(ev[i + 1], stream_);

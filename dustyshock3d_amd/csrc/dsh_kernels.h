@@ -66,7 +66,6 @@ hipError_t launch_key(const StepArgs& a, hipStream_t s);
 hipError_t launch_scan(const StepArgs& a, hipStream_t s);
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s);
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s);
-hipError_t launch_cellmax(const StepArgs& a, hipStream_t s);
 hipError_t launch_radius(const StepArgs& a, hipStream_t s);
 hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s);
 hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s);

@@ -5,9 +5,9 @@
 //   key          neighbour-cell key per particle; wave-aggregated atomic slot in its cell
 //   scan         exclusive scan of cell counts -> cell starts (global over all segments)
 //   scatter      particle -> (cell start + slot)
-//   rank_permute rank inside the cell by particle id, gather the state into the work buffer
-//                (the sorted order is (cell key, id): the reference's linked-list order)
-//   cellmax      max h per cell, per (species, tube)
+//   rank_permute rank inside the cell by (x, id), gather the state into the work buffer,
+//                max h per cell and per (species, tube); the sorted order is (cell key, x,
+//                id), and the reference's linked lists (cell, id) are rebuilt at download
 //   radius       per-cell search radius in cells, from a window max of cell h
 //   density      same-species density sums; pressure, sound speed, stopping time
 //   force        gas pressure + artificial-viscosity acceleration and de/dt
@@ -461,19 +461,49 @@ __global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__
 }
 
 
+// Largest h per neighbour cell and per (tube, species), one thread per position of the
+// block (called by every thread): keys ascend along a wave, so a segmented max over runs of
+// equal keys leaves each run's maximum on its last lane, which raises the cell's value
+// (h > 0, so the FP64 bits order like the values).  Cells without particles keep the 0
+// written by k_begin.
+__device__ __forceinline__ void cell_max_h(const ItemPos& ip, bool valid, int k, double h,
+                                           double* __restrict__ cellh, TubeDyn* __restrict__ td) {
+    __shared__ double wmax[kBS / 64];
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const double ho = __shfl_up(h, o, 64);
+        const int ko = __shfl_up(k, o, 64);
+        if (lane >= o && ko == k) h = fmax(h, ho);
+    }
+    const int kn = __shfl_down(k, 1, 64);
+    if (valid && (lane == 63 || kn != k))
+        atomicMax(reinterpret_cast<unsigned long long*>(cellh + k), (unsigned long long)__double_as_longlong(h));
+    const double m = wave_max(h);
+    if (lane == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bm = 0.0;
+        for (int w = 0; w < kBS / 64; ++w) bm = fmax(bm, wmax[w]);
+        unsigned long long* t = &td[ip.sg.tube].hmax_bits[ip.sg.species];
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(bm);
+        // most blocks do not raise the tube maximum: read before the atomic
+        if (bm > 0.0 && bits > __atomic_load_n(t, __ATOMIC_RELAXED)) atomicMax(t, bits);
+    }
+}
+
 constexpr int kRankRun = 1024;   // cell members staged per block by k_rank_permute
 
 // Final position = cell start + number of cell members before it in (x, id) order: cells
 // are sorted by x, so a row of cells is one x-sorted run for the pair kernels' windows.
 // A finished tube is still sorted every step (its positions moved on its last drift, so
 // the order can change once); its last acceleration and de/dt move with it.
-__global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __restrict__ td,
+__global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restrict__ td,
                                                       const int* __restrict__ cur, PBuf B,
                                                       const int* __restrict__ start, const int* __restrict__ tsrc,
                                                       const int* __restrict__ tid, const int* __restrict__ tkey,
                                                       const unsigned long long* __restrict__ tx,
-                                                      int* __restrict__ wkey,
-                                                      int* __restrict__ ginv) {
+                                                      int* __restrict__ wkey, int* __restrict__ ginv,
+                                                      double* __restrict__ cellh) {
     // the block's positions lie in one segment and keys ascend along them, so the members of
     // every cell the block touches form one run [w0, w1), staged in LDS when it fits
     __shared__ unsigned long long sx[kRankRun];
@@ -491,9 +521,12 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
         }
     }
     __syncthreads();
-    if (li >= ip.sg.n) return;
+    const bool valid = li < ip.sg.n;
+    int c = -1;
+    double hq = 0.0;
+    if (valid) {
     const int q = ip.sg.off + li;
-    const int c = tkey[q];
+    c = tkey[q];
     const int lo = start[c], hi = start[c + 1];
     const int my = tid[q];
     const unsigned long long mx = tx[q];
@@ -517,7 +550,8 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
     B.vy[wo + dst] = B.vy[so + src];
     B.vz[wo + dst] = B.vz[so + src];
     B.m[wo + dst] = B.m[so + src];
-    B.h[wo + dst] = B.h[so + src];
+    hq = B.h[so + src];
+    B.h[wo + dst] = hq;
     const int ind = B.ind[so + src];
     B.ind[wo + dst] = ind;
     B.id[wo + dst] = my;
@@ -539,43 +573,8 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, const TubeDyn* __r
         B.ts[wo + dst] = B.ts[so + src];
     }
     wkey[dst] = c;
-}
-
-// Largest h per neighbour cell and per (tube, species), one thread per sorted particle:
-// keys ascend along a wave, so a segmented max over runs of equal keys leaves each run's
-// maximum on its last lane, which raises the cell's value (h > 0, so the FP64 bits order
-// like the values).  Cells without particles keep the 0 written by k_begin.
-__global__ __launch_bounds__(kBS) void k_cellmax(Map mp, const int* __restrict__ cur, PBuf B,
-                                                 const int* __restrict__ wkey, double* __restrict__ cellh,
-                                                 TubeDyn* __restrict__ td) {
-    __shared__ double wmax[kBS / 64];
-    const ItemPos ip = locate(mp, kBS);
-    const int li = ip.li + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const bool valid = li < ip.sg.n;
-    const long long wo = (long long)(1 - *cur) * B.ntot;
-    const int q = ip.sg.off + li;
-    const int k = valid ? wkey[q] : -1;
-    double h = valid ? B.h[wo + q] : 0.0;
-    for (int o = 1; o < 64; o <<= 1) {
-        const double ho = __shfl_up(h, o, 64);
-        const int ko = __shfl_up(k, o, 64);
-        if (lane >= o && ko == k) h = fmax(h, ho);
     }
-    const int kn = __shfl_down(k, 1, 64);
-    if (valid && (lane == 63 || kn != k))
-        atomicMax(reinterpret_cast<unsigned long long*>(cellh + k), (unsigned long long)__double_as_longlong(h));
-    const double m = wave_max(h);
-    if (lane == 0) wmax[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double bm = 0.0;
-        for (int w = 0; w < kBS / 64; ++w) bm = fmax(bm, wmax[w]);
-        unsigned long long* t = &td[ip.sg.tube].hmax_bits[ip.sg.species];
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(bm);
-        // most blocks do not raise the tube maximum: read before the atomic
-        if (bm > 0.0 && bits > __atomic_load_n(t, __ATOMIC_RELAXED)) atomicMax(t, bits);
-    }
+    cell_max_h(ip, valid, c, hq, cellh, td);
 }
 
 // Reach of a cell: every neighbour j of a particle i of the cell has |x_i - x_j| < h_i + h_j
@@ -1805,12 +1804,7 @@ hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
 
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.cur, a.B, a.start,
-                       a.tsrc, a.tid, a.tkey, a.tx, a.wkey, a.ginv);
-    return hipGetLastError();
-}
-
-hipError_t launch_cellmax(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_cellmax, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.wkey, a.cellh, a.td);
+                       a.tsrc, a.tid, a.tkey, a.tx, a.wkey, a.ginv, a.cellh);
     return hipGetLastError();
 }
 
