@@ -76,6 +76,18 @@ def measured_traffic(workload, stage, tile=160):
     return sum(kern[n]["read_bytes"] + kern[n]["write_bytes"] for n in names)
 
 
+def measured_valu(workload, stage, tile=160):
+    """VALU issue fraction of a pass's kernel from the committed SQ-counter summary (or None)."""
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    kern = json.load(open(path))["kernels"]
+    names = pass_kernels(stage, tile)
+    if len(names) != 1 or names[0] not in kern:
+        return None
+    return kern[names[0]]["valu_issue_frac"]
+
+
 def cpu_baseline(spec, threads, steps=1):
     """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube."""
     import oracle
@@ -180,7 +192,11 @@ def main():
                      "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else 160)
                      if args.variant == 1 else None,
                      "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
-                     "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms},
+                     "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms,
+                     # the pair passes are compute-bound: the share of VALU issue slots they use
+                     "valu_issue_frac": measured_valu(args.config, dominant, tune["tile"] if tune else 160)
+                     if args.variant == 1 else None,
+                     "valu_source": f"profiles/valu_{args.config}.json (rocprofv3 SQ_INSTS_VALU)"},
         "step_roofline": {"algo_bytes": step_bytes, "event_ms": step_ms,
                           "frac": step_bytes / (wall / max(steps_done, 1)) / 1e9 / HBM_PEAK_GBS},
         "stages_ms": {k: round(v, 4) for k, v in kt.items()},
