@@ -235,7 +235,11 @@ int dsh_tube_params(const dsh_tube_spec* t, dsh_params* p) {
     p->dt_mode = t->dt_mode;
     p->cfl = t->cfl;
     p->max_steps = t->max_steps;
-    p->avg_factor = 2;
+    // averaging cell = one neighbour cell (2.1 h_min, ~16 particles per species on the dense
+    // lattice): close to the ~8 per species of the reference's own run (0.5 h with h = 4
+    // spacings), and the width whose shock-tube solution is closest to the isothermal
+    // Riemann limits (coupled c_s/sqrt(1+eps), decoupled c_s)
+    p->avg_factor = 1;
     p->precision = t->precision;
     return DSH_OK;
 }

@@ -1491,10 +1491,141 @@ __device__ __forceinline__ void for_avg_cell(const TubeConst& T, const int* __re
     }
 }
 
-// One wave per averaging cell: sums over its f^3 neighbour cells for every species,
-// implicit drag solve, then the per-particle kick (FP64), drift, energy and h update,
-// and the cell's conservation / CFL partials.  A gas-only tube takes the explicit kick
-// v += A tau of the plain scheme instead of the drag solve.
+// Sums over one averaging cell: gas count, v (3), A (3); per dust species count, v (3), t_stop.
+struct CellSums {
+    int ng;
+    double sg[6];
+    int ndc[kMaxDust];
+    double sd[kMaxDust][4];
+};
+// What the per-particle update of one averaging cell needs.
+struct CellDrag {
+    double psi[3], vgh[3], vdh[kMaxDust][3], tb[kMaxDust], tmp1, tmpv[3];
+};
+
+// Means, fallbacks (kernel.cu:2126-2211), the implicit solve (kernel.cu:2294-2437) and the
+// gas velocity terms (IDIC.h:301-309) of one averaging cell.
+__device__ __forceinline__ CellDrag cell_drag(const TubeConst& T, int nd, double tau, const CellSums& S) {
+    CellDrag R;
+    double vg[3], vdm[kMaxDust][3], ep[kMaxDust];
+    for (int a = 0; a < 3; ++a) {
+        vg[a] = S.ng > 0 ? S.sg[a] / S.ng : S.sg[a];
+        R.psi[a] = S.ng > 0 ? S.sg[3 + a] / S.ng : S.sg[3 + a];
+    }
+    // loops over dust species unrolled so that the arrays stay in registers
+#pragma unroll
+    for (int k = 0; k < kMaxDust; ++k) {
+        vdm[k][0] = vdm[k][1] = vdm[k][2] = 0.0;
+        R.tb[k] = 1.0;
+        ep[k] = 0.0;
+        if (k >= nd) continue;
+        for (int a = 0; a < 3; ++a) vdm[k][a] = S.ndc[k] > 0 ? S.sd[k][a] / S.ndc[k] : S.sd[k][a];
+        R.tb[k] = S.ndc[k] > 0 ? S.sd[k][3] / S.ndc[k] : T.t_fb;
+        const int test = (k == 1 && T.eps2_quirk) ? S.ndc[0] : S.ndc[k];
+        ep[k] = (S.ng > 0 && test > 0) ? S.ndc[k] * T.mu[k] / S.ng : T.eps_fb[k];
+    }
+    for (int a = 0; a < 3; ++a) {
+        R.vgh[a] = 0.0;
+        for (int k = 0; k < kMaxDust; ++k) R.vdh[k][a] = 0.0;
+    }
+    if (nd > 0)
+        for (int a = 0; a < 3; ++a) {
+            double vdc[kMaxDust] = {vdm[0][a], nd > 1 ? vdm[1][a] : 0.0};
+            double out[kMaxDust] = {0.0, 0.0};
+            idic_solve(nd, vg[a], vdc, R.psi[a], R.tb, ep, tau, &R.vgh[a], out);
+#pragma unroll
+            for (int k = 0; k < kMaxDust; ++k)
+                if (k < nd) R.vdh[k][a] = out[k];
+        }
+    R.tmp1 = 0.0;
+    R.tmpv[0] = R.tmpv[1] = R.tmpv[2] = 0.0;
+    if (nd == 1) {
+        R.tmp1 = R.tmp1 + ep[0] / R.tb[0];
+        for (int a = 0; a < 3; ++a) R.tmpv[a] = R.tmpv[a] + ep[0] * R.vdh[0][a] / R.tb[0];
+    } else if (nd == 2) {
+        R.tmp1 = R.tmp1 + ep[0] / R.tb[0] + ep[1] / R.tb[1];
+        for (int a = 0; a < 3; ++a)
+            R.tmpv[a] = R.tmpv[a] + ep[0] * R.vdh[0][a] / R.tb[0] + ep[1] * R.vdh[1][a] / R.tb[1];
+    }
+    // gas only: the explicit kick of MoveKernel in particle_update
+    return R;
+}
+
+// Kick (implicit drag, IDIC.h:264-315, or the explicit v += A tau of a gas-only run), drift
+// with periodic wrap, energy, smoothing length (IDIC.h:319-346) of sorted particle g of
+// species s, and its conservation / CFL contributions.
+__device__ __forceinline__ void particle_update(const TubeConst& T, const PBuf& B, long long g, int s, int nd,
+                                                double tau, const CellDrag& C, double& px, double& py, double& pz,
+                                                double& E, double& mn) {
+    double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
+    const double m = B.m[g];
+    double e = 0.0;
+    if (B.ind[g] == 0) {
+        if (s == 0 && nd == 0) {
+            // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
+            vx = vx + B.ax[g] * tau;
+            vy = vy + B.ay[g] * tau;
+            vz = vz + B.az[g] * tau;
+        } else if (s == 0) {
+            vx = (vx / tau + C.tmpv[0] + C.psi[0]) / (1.0 / tau + C.tmp1);
+            vy = (vy / tau + C.tmpv[1] + C.psi[1]) / (1.0 / tau + C.tmp1);
+            vz = (vz / tau + C.tmpv[2] + C.psi[2]) / (1.0 / tau + C.tmp1);
+        } else {
+            const double tk = C.tb[s - 1];
+            vx = (vx / tau + C.vgh[0] / tk) / (1.0 / tau + 1.0 / tk);
+            vy = (vy / tau + C.vgh[1] / tk) / (1.0 / tau + 1.0 / tk);
+            vz = (vz / tau + C.vgh[2] / tk) / (1.0 / tau + 1.0 / tk);
+        }
+        B.vx[g] = vx;
+        B.vy[g] = vy;
+        B.vz[g] = vz;
+        double x = B.x[g] + vx * tau;
+        double y = B.y[g] + vy * tau;
+        double z = B.z[g] + vz * tau;
+        if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
+        if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
+        B.x[g] = x;
+        B.y[g] = y;
+        B.z[g] = z;
+        if (s == 0) {
+            e = B.e[g] + B.ed[g] * tau;
+            B.e[g] = e;
+        }
+        double h = B.h[g];
+        if (T.h_mode == 1) {
+            h = T.eta * cbrt(m / B.rho[g]);
+            B.h[g] = h;
+        }
+        const double v = sqrt(vx * vx + vy * vy + vz * vz);
+        mn = fmin(mn, h / (T.cs + v));
+    } else if (s == 0) {
+        e = B.e[g];
+    }
+    px += m * vx;
+    py += m * vy;
+    pz += m * vz;
+    double kin = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    if (s == 0) kin += m * e;
+    E += kin;
+}
+
+// t_stop of dust particle k for drag_mode 0 (t_stop_Kernel_IDIC, IDIC.h:78-91): the gas
+// particle with the same index, this step's pressure and density; written back.
+__device__ __forceinline__ double dust_tstop(const TubeConst& T, const PBuf& B, const int* __restrict__ ginv,
+                                             long long wo, int k, int s) {
+    double ts = B.ts[wo + k];
+    if (T.drag_mode == 0 && B.ind[wo + k] == 0) {
+        const long long gq = wo + ginv[T.p_off[0] + B.id[wo + k]];
+        const double rg = B.rho[gq];
+        ts = T.R[s - 1] / (sqrt(T.gamma * B.p[gq] / rg) * rg);
+        B.ts[wo + k] = ts;
+    }
+    return ts;
+}
+
+// Averaging cells of f^3 neighbour cells (f >= 2): one wave per averaging cell, sums over
+// its neighbour cells for every species, implicit drag solve, then the per-particle kick,
+// drift, energy and h update, and the cell's conservation / CFL partials.
 __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restrict__ tc,
                                               const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                               const int* __restrict__ start, const int* __restrict__ ginv,
@@ -1507,6 +1638,7 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     const TubeDyn& D = td[t];
     if (D.done || D.err) return;
     const TubeConst& T = tc[t];
+    if (T.f == 1) return;   // k_idic_runs
     const long long wo = (long long)(1 - *cur) * B.ntot;
     const int ax_ = la % T.na[0];
     const int ay_ = (la / T.na[0]) % T.na[1];
@@ -1514,134 +1646,42 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     const int nd = T.nsp - 1;
     const double tau = D.dt;
 
-    // sums: gas count, v (3), A (3); dust count, v (3), t_stop
-    int ng = 0;
-    double sg[6] = {0, 0, 0, 0, 0, 0};
-    int ndc[kMaxDust] = {0, 0};
-    double sd[kMaxDust][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    for (int s = 0; s < T.nsp; ++s) {
+    CellSums S{};
+    // species loops unrolled: the per-species arrays are then indexed by constants
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        if (s >= T.nsp) break;
         for_avg_cell(T, start, s, ax_, ay_, az_, lane, [&](int k) {
             if (s == 0) {
-                ++ng;
-                sg[0] += B.vx[wo + k];
-                sg[1] += B.vy[wo + k];
-                sg[2] += B.vz[wo + k];
-                sg[3] += B.ax[wo + k];
-                sg[4] += B.ay[wo + k];
-                sg[5] += B.az[wo + k];
+                ++S.ng;
+                S.sg[0] += B.vx[wo + k];
+                S.sg[1] += B.vy[wo + k];
+                S.sg[2] += B.vz[wo + k];
+                S.sg[3] += B.ax[wo + k];
+                S.sg[4] += B.ay[wo + k];
+                S.sg[5] += B.az[wo + k];
             } else {
-                ndc[s - 1]++;
-                sd[s - 1][0] += B.vx[wo + k];
-                sd[s - 1][1] += B.vy[wo + k];
-                sd[s - 1][2] += B.vz[wo + k];
-                double ts = B.ts[wo + k];
-                if (T.drag_mode == 0 && B.ind[wo + k] == 0) {
-                    // t_stop_Kernel_IDIC (IDIC.h:78-91): the gas particle with the same
-                    // index, this step's pressure and density
-                    const long long gq = wo + ginv[T.p_off[0] + B.id[wo + k]];
-                    const double rg = B.rho[gq];
-                    ts = T.R[s - 1] / (sqrt(T.gamma * B.p[gq] / rg) * rg);
-                    B.ts[wo + k] = ts;
-                }
-                sd[s - 1][3] += ts;
+                S.ndc[s - 1]++;
+                S.sd[s - 1][0] += B.vx[wo + k];
+                S.sd[s - 1][1] += B.vy[wo + k];
+                S.sd[s - 1][2] += B.vz[wo + k];
+                S.sd[s - 1][3] += dust_tstop(T, B, ginv, wo, k, s);
             }
         });
     }
-    ng = wave_sum_i(ng);
-    for (int a = 0; a < 6; ++a) sg[a] = wave_sum(sg[a]);
+    S.ng = wave_sum_i(S.ng);
+    for (int a = 0; a < 6; ++a) S.sg[a] = wave_sum(S.sg[a]);
     for (int k = 0; k < nd; ++k) {
-        ndc[k] = wave_sum_i(ndc[k]);
-        for (int a = 0; a < 4; ++a) sd[k][a] = wave_sum(sd[k][a]);
+        S.ndc[k] = wave_sum_i(S.ndc[k]);
+        for (int a = 0; a < 4; ++a) S.sd[k][a] = wave_sum(S.sd[k][a]);
     }
-    // means and fallbacks (kernel.cu:2126-2211)
-    double vg[3], psi[3], vdm[kMaxDust][3], tb[kMaxDust], ep[kMaxDust];
-    for (int a = 0; a < 3; ++a) {
-        vg[a] = ng > 0 ? sg[a] / ng : sg[a];
-        psi[a] = ng > 0 ? sg[3 + a] / ng : sg[3 + a];
-    }
-    for (int k = 0; k < nd; ++k) {
-        for (int a = 0; a < 3; ++a) vdm[k][a] = ndc[k] > 0 ? sd[k][a] / ndc[k] : sd[k][a];
-        tb[k] = ndc[k] > 0 ? sd[k][3] / ndc[k] : T.t_fb;
-    }
-    for (int k = 0; k < nd; ++k) {
-        const int test = (k == 1 && T.eps2_quirk) ? ndc[0] : ndc[k];
-        ep[k] = (ng > 0 && test > 0) ? ndc[k] * T.mu[k] / ng : T.eps_fb[k];
-    }
-    double vgh[3] = {0.0, 0.0, 0.0}, vdh[kMaxDust][3];
-    if (nd > 0)
-        for (int a = 0; a < 3; ++a) {
-            double vdc[kMaxDust] = {vdm[0][a], nd > 1 ? vdm[1][a] : 0.0};
-            double out[kMaxDust] = {0.0, 0.0};
-            idic_solve(nd, vg[a], vdc, psi[a], tb, ep, tau, &vgh[a], out);
-            for (int k = 0; k < nd; ++k) vdh[k][a] = out[k];
-        }
-    // gas velocity terms (IDIC.h:301-309)
-    double tmp1 = 0.0, tmpv[3] = {0.0, 0.0, 0.0};
-    if (nd == 0) {
-        // gas only: the explicit kick of MoveKernel below
-    } else if (nd == 1) {
-        tmp1 = tmp1 + ep[0] / tb[0];
-        for (int a = 0; a < 3; ++a) tmpv[a] = tmpv[a] + ep[0] * vdh[0][a] / tb[0];
-    } else {
-        tmp1 = tmp1 + ep[0] / tb[0] + ep[1] / tb[1];
-        for (int a = 0; a < 3; ++a) tmpv[a] = tmpv[a] + ep[0] * vdh[0][a] / tb[0] + ep[1] * vdh[1][a] / tb[1];
-    }
-    // per-particle update
+    const CellDrag C = cell_drag(T, nd, tau, S);
     double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
-    for (int s = 0; s < T.nsp; ++s) {
-        for_avg_cell(T, start, s, ax_, ay_, az_, lane, [&](int k) {
-            const long long g = wo + k;
-            double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
-            const double m = B.m[g];
-            double e = 0.0;
-            if (B.ind[g] == 0) {
-                if (s == 0 && nd == 0) {
-                    // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
-                    vx = vx + B.ax[g] * tau;
-                    vy = vy + B.ay[g] * tau;
-                    vz = vz + B.az[g] * tau;
-                } else if (s == 0) {
-                    vx = (vx / tau + tmpv[0] + psi[0]) / (1.0 / tau + tmp1);
-                    vy = (vy / tau + tmpv[1] + psi[1]) / (1.0 / tau + tmp1);
-                    vz = (vz / tau + tmpv[2] + psi[2]) / (1.0 / tau + tmp1);
-                } else {
-                    const double tk = tb[s - 1];
-                    vx = (vx / tau + vgh[0] / tk) / (1.0 / tau + 1.0 / tk);
-                    vy = (vy / tau + vgh[1] / tk) / (1.0 / tau + 1.0 / tk);
-                    vz = (vz / tau + vgh[2] / tk) / (1.0 / tau + 1.0 / tk);
-                }
-                B.vx[g] = vx;
-                B.vy[g] = vy;
-                B.vz[g] = vz;
-                double x = B.x[g] + vx * tau;
-                double y = B.y[g] + vy * tau;
-                double z = B.z[g] + vz * tau;
-                if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
-                if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
-                B.x[g] = x;
-                B.y[g] = y;
-                B.z[g] = z;
-                if (s == 0) {
-                    e = B.e[g] + B.ed[g] * tau;
-                    B.e[g] = e;
-                }
-                double h = B.h[g];
-                if (T.h_mode == 1) {
-                    h = T.eta * cbrt(m / B.rho[g]);
-                    B.h[g] = h;
-                }
-                const double v = sqrt(vx * vx + vy * vy + vz * vz);
-                mn = fmin(mn, h / (T.cs + v));
-            } else if (s == 0) {
-                e = B.e[g];
-            }
-            px += m * vx;
-            py += m * vy;
-            pz += m * vz;
-            double kin = 0.5 * m * (vx * vx + vy * vy + vz * vz);
-            if (s == 0) kin += m * e;
-            E += kin;
-        });
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        if (s >= T.nsp) break;
+        for_avg_cell(T, start, s, ax_, ay_, az_, lane,
+                     [&](int k) { particle_update(T, B, wo + k, s, nd, tau, C, px, py, pz, E, mn); });
     }
     px = wave_sum(px);
     py = wave_sum(py);
@@ -1650,6 +1690,105 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     mn = wave_min(mn);
     if (lane == 0) {
         double* o = part + 5ll * (T.avg_off + la);
+        o[0] = px;
+        o[1] = py;
+        o[2] = pz;
+        o[3] = E;
+        o[4] = mn;
+    }
+}
+
+// Averaging cell = one neighbour cell (f == 1): averaging cell la is neighbour cell la, so a
+// run of kRunCells consecutive cells is one contiguous range of sorted particles per species.
+// One wave per run: lanes take 64 particles at a time; their contributions go to LDS and lane
+// j sums cell j's in sorted order, solves cell j and leaves its drag terms in LDS for the
+// particles of the cell.  One conservation / CFL partial per run.
+constexpr int kRunCells = 8;
+__global__ __launch_bounds__(kBS) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
+                                                   const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+                                                   const int* __restrict__ start, const int* __restrict__ ginv,
+                                                   double* __restrict__ part) {
+    __shared__ double sv[kBS / 64][6][64];
+    __shared__ CellDrag cd[kBS / 64][kRunCells];
+    const ItemPos ip = locate(mp, (kBS / 64) * kRunCells);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int la0 = ip.li + wave * kRunCells;
+    if (la0 >= ip.sg.n) return;
+    const int t = ip.sg.tube;
+    const TubeDyn& D = td[t];
+    if (D.done || D.err) return;
+    const TubeConst& T = tc[t];
+    if (T.f != 1) return;   // k_idic
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    const int ncl = min(kRunCells, ip.sg.n - la0);
+    const int nd = T.nsp - 1;
+    const double tau = D.dt;
+    double (*V)[64] = sv[wave];
+
+    // lane j < ncl: sums of cell la0 + j
+    CellSums S{};
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        if (s >= T.nsp) break;
+        const int c0 = T.cell_off[s] + la0;
+        const int p0 = start[c0], p1 = start[c0 + ncl];
+        const int jlo = lane < ncl ? start[c0 + lane] : 0, jhi = lane < ncl ? start[c0 + lane + 1] : 0;
+        if (lane < ncl) {
+            if (s == 0) S.ng = jhi - jlo;
+            else S.ndc[s - 1] = jhi - jlo;
+        }
+        for (int base = p0; base < p1; base += 64) {
+            const int k = base + lane;
+            if (k < p1) {
+                const long long g = wo + k;
+                V[0][lane] = B.vx[g];
+                V[1][lane] = B.vy[g];
+                V[2][lane] = B.vz[g];
+                if (s == 0) {
+                    V[3][lane] = B.ax[g];
+                    V[4][lane] = B.ay[g];
+                    V[5][lane] = B.az[g];
+                } else {
+                    V[3][lane] = dust_tstop(T, B, ginv, wo, k, s);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < ncl) {
+                const int a = max(jlo, base) - base, e = min(jhi, base + 64) - base;
+                for (int u = a; u < e; ++u) {
+                    if (s == 0) {
+                        for (int c = 0; c < 6; ++c) S.sg[c] += V[c][u];
+                    } else {
+                        for (int c = 0; c < 4; ++c) S.sd[s - 1][c] += V[c][u];
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+    if (lane < ncl) cd[wave][lane] = cell_drag(T, nd, tau, S);
+    __builtin_amdgcn_wave_barrier();
+
+    double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
+#pragma unroll
+    for (int s = 0; s < kMaxSpecies; ++s) {
+        if (s >= T.nsp) break;
+        const int c0 = T.cell_off[s] + la0;
+        const int p0 = start[c0], p1 = start[c0 + ncl];
+        for (int k = p0 + lane; k < p1; k += 64) {
+            // the cell of k: the last cell of the run whose start is <= k
+            int j = 0;
+            for (int jj = 1; jj < ncl; ++jj) j = (start[c0 + jj] <= k) ? jj : j;
+            particle_update(T, B, wo + k, s, nd, tau, cd[wave][j], px, py, pz, E, mn);
+        }
+    }
+    px = wave_sum(px);
+    py = wave_sum(py);
+    pz = wave_sum(pz);
+    E = wave_sum(E);
+    mn = wave_min(mn);
+    if (lane == 0) {
+        double* o = part + 5ll * (T.avg_off + la0 / kRunCells);
         o[0] = px;
         o[1] = py;
         o[2] = pz;
@@ -1674,7 +1813,7 @@ __global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict
     const bool failed = D.err != 0;
     double a[4] = {0, 0, 0, 0}, mn = 1e300;
     if (!failed) {
-        for (int i = threadIdx.x; i < T.navg; i += kFinBS) {
+        for (int i = threadIdx.x; i < T.npart; i += kFinBS) {
             const double* o = part + 5ll * (T.avg_off + i);
             for (int c = 0; c < 4; ++c) a[c] += o[c];
             mn = fmin(mn, o[4]);
@@ -1884,10 +2023,16 @@ hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
-                       a.ginv, a.part);
+    if (a.idic_blocks)
+        hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
+                           a.ginv, a.part);
+    if (a.idic_runs)
+        hipLaunchKernelGGL(k_idic_runs, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
+                           a.B, a.start, a.ginv, a.part);
     return hipGetLastError();
 }
+
+int idic_run_cells() { return kRunCells; }
 
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kFinBS), 0, s, a.tc, a.td, a.cur, a.part);

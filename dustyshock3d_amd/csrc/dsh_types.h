@@ -31,7 +31,8 @@ struct TubeConst {
     int na[3];         // averaging cells per axis
     int ncell;         // n0 n1 n2
     int navg;
-    int avg_off;       // this tube's averaging cells in the batch arrays
+    int avg_off;       // this tube's averaging cells (and conservation partials) in the batch arrays
+    int npart;         // conservation partials: navg, or one per run of cells when f == 1
     int nsp;           // 1 + n_dust
     int cell_off[kMaxSpecies];
     int p_off[kMaxSpecies];

@@ -88,7 +88,7 @@ typedef struct dsh_params {
     int    max_steps;             /* 0: no step limit */
     double cell_size;             /* 0: automatic, 2 h_min (1 + cell_slack) */
     double cell_slack;
-    int    avg_factor;            /* averaging cell = avg_factor^3 neighbour cells */
+    int    avg_factor;            /* averaging cell = avg_factor^3 neighbour cells (1 for the shock tubes) */
     int    precision;             /* DSH_FP64 or DSH_MIXED */
 } dsh_params;
 

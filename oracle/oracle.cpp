@@ -939,7 +939,7 @@ int orc_tube_params(const dsh_tube_spec* t, dsh_params* p) {
     p->max_steps = t->max_steps;
     p->cell_size = 0.0;
     p->cell_slack = 0.05;
-    p->avg_factor = 2;
+    p->avg_factor = 1;
     p->precision = t->precision;
     return DSH_OK;
 }
