@@ -28,14 +28,16 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per particle of each pass (compulsory traffic, cell-sorted SoA):
 # (gas, dust).  The idic stage fuses the cell reduction (60 / 52) with the drag +
-# kick-drift + energy + conservation pass (108 / 96) and adds 352 B per averaging cell.
+# kick-drift + energy + conservation pass (108 / 96).  Per averaging cell of one neighbour
+# cell it adds the cell start of each species (2 x 4 B) and a 40 B conservation / CFL
+# partial per run of 8 cells (5 B): 13 B.
 ALGO_BYTES = {
     "sort": (144, 136),
     "density": (28, 28),
     "force": (68, 0),
     "idic": (168, 148),
 }
-AVG_CELL_BYTES = 352
+AVG_CELL_BYTES = 13
 SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "radius")
 
 
@@ -58,7 +60,7 @@ def pass_kernels(stage, tile=160):
     return {
         "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
         "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
-        "idic": ["k_idic"],
+        "idic": ["k_idic_runs"],
         "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
                  "k_window_max", "k_radius"],
     }.get(stage, [])
