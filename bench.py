@@ -337,7 +337,8 @@ def serve_mode(args):
         "serial_fifo_time_to_solution_s": {"p50": float(np.percentile(serial_tts, 50)),
                                            "p99": float(np.percentile(serial_tts, 99))},
         "serial_particle_steps_per_s": float(psteps.sum() / service.sum()),
-        "wall_s": wall, "rebuilds": st["rebuilds"], "epochs": st["epochs"], "hbm_budget": args.budget,
+        "wall_s": wall, "rebuilds": st["rebuilds"], "epochs": st["epochs"],
+        "rebuild_s": st["rebuild_s"], "advance_s": st["advance_s"], "hbm_budget": args.budget,
         "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
         "config": {"workload": "cfg4: Poisson arrivals, 95% cfg3-distribution tubes, 5% large tubes "
                                "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch},

@@ -159,7 +159,10 @@ class Server:
         b = C.c_double()
         r, e = C.c_int(), C.c_int()
         self._call(self._lib.dsh_server_stats(self._h, C.byref(b), C.byref(r), C.byref(e)))
-        return {"bytes_in_use": b.value, "rebuilds": r.value, "epochs": e.value}
+        rs, adv = C.c_double(), C.c_double()
+        self._call(self._lib.dsh_server_times(self._h, C.byref(rs), C.byref(adv)))
+        return {"bytes_in_use": b.value, "rebuilds": r.value, "epochs": e.value,
+                "rebuild_s": rs.value, "advance_s": adv.value}
 
     def close(self):
         if self._h:

@@ -4,10 +4,14 @@
 //   1. admit queued requests in arrival order while the new batch and the old one
 //      (both live during the rebuild) fit the HBM budget; a request larger than the
 //      budget is rejected, otherwise the queue waits at the first that does not fit;
-//   2. if the set changed, build a new batch: running tubes are adopted device to
-//      device (state, sorted order, clock), new tubes get their seeded set-up;
+//   2. if requests were admitted or cancelled, build a new batch: running tubes are
+//      adopted device to device (state, sorted order, clock), new tubes get their
+//      seeded set-up, finished tubes are dropped;
 //   3. advance the batch steps_per_epoch steps (one graph launch per step);
-//   4. retire finished tubes: copy their arrays to host memory and free their slots.
+//   4. retire finished tubes: copy their arrays to host memory.  Their slots stay in
+//      the batch (the kernels skip a finished tube) until the next rebuild, so a tube
+//      that finishes costs no rebuild of its own; when none is left running the batch
+//      is released.
 // The device behind this is a Backend: the GPU engine, or a host-only fake used to
 // test the scheduling logic without a GPU.
 #include <algorithm>
@@ -240,8 +244,8 @@ public:
                 keep.push_back((int)s);
                 keep_ids.push_back(r.id);
                 keep_bytes += r.bytes;
-            } else {
-                changed = true;
+            } else if (r.state == DSH_REQ_CANCELLED) {
+                changed = true;   // still stepped by the device: drop it now
             }
         }
         std::vector<const Request*> add;
@@ -262,7 +266,9 @@ public:
         }
         if (!add.empty()) changed = true;
         if (changed) {
+            const double t0 = wall_clock(nullptr);
             const int rc = be_->rebuild(keep, add);
+            rebuild_s_ += wall_clock(nullptr) - t0;
             ++rebuilds_;
             if (rc != DSH_OK) {
                 // the old batch keeps running; the new requests go back to the queue head
@@ -281,7 +287,9 @@ public:
         }
         // 3: advance
         if (!slot_ids_.empty()) {
+            const double t0 = wall_clock(nullptr);
             const int rc = be_->advance(o_.steps_per_epoch);
+            advance_s_ += wall_clock(nullptr) - t0;
             ++epochs_;
             if (rc) return fail(rc, be_->error());
         }
@@ -305,6 +313,15 @@ public:
         }
         int nr = 0;
         for (uint64_t id : slot_ids_) nr += reqs_[id].state == DSH_REQ_RUNNING ? 1 : 0;
+        if (nr == 0 && !slot_ids_.empty()) {
+            // nothing left running: release the batch's device memory
+            const double t0 = wall_clock(nullptr);
+            const int rc = be_->rebuild({}, {});
+            rebuild_s_ += wall_clock(nullptr) - t0;
+            ++rebuilds_;
+            if (rc) return fail(rc, be_->error());
+            slot_ids_.clear();
+        }
         if (running) *running = nr;
         if (queued) {
             int nq = 0;
@@ -330,6 +347,8 @@ public:
     double bytes_in_use() const { return be_->bytes(); }
     int rebuilds() const { return rebuilds_; }
     int epochs() const { return epochs_; }
+    double rebuild_seconds() const { return rebuild_s_; }
+    double advance_seconds() const { return advance_s_; }
     std::string err_;
 
 private:
@@ -344,6 +363,7 @@ private:
     std::vector<uint64_t> slot_ids_;
     uint64_t next_id_ = 1;
     int rebuilds_ = 0, epochs_ = 0;
+    double rebuild_s_ = 0, advance_s_ = 0;
 };
 
 }  // namespace dsh
@@ -434,6 +454,13 @@ int dsh_server_stats(dsh_server* s, double* bytes_in_use, int* rebuilds, int* ep
     if (bytes_in_use) *bytes_in_use = s->sv->bytes_in_use();
     if (rebuilds) *rebuilds = s->sv->rebuilds();
     if (epochs) *epochs = s->sv->epochs();
+    return DSH_OK;
+}
+
+int dsh_server_times(dsh_server* s, double* rebuild_s, double* advance_s) {
+    if (!s) return DSH_ERR_ARG;
+    if (rebuild_s) *rebuild_s = s->sv->rebuild_seconds();
+    if (advance_s) *advance_s = s->sv->advance_seconds();
     return DSH_OK;
 }
 

@@ -251,6 +251,9 @@ int dsh_server_result(dsh_server* sv, uint64_t id, int species, const dsh_soa_vi
 int dsh_server_cancel(dsh_server* sv, uint64_t id);
 int dsh_server_footprint(const dsh_tube_spec* spec, double* bytes);
 int dsh_server_stats(dsh_server* sv, double* bytes_in_use, int* rebuilds, int* epochs);
+/* Host wall-clock seconds spent in batch rebuilds (allocation, adoption, set-up) and in
+   advancing epochs (step launches and the wait for them), since creation. */
+int dsh_server_times(dsh_server* sv, double* rebuild_s, double* advance_s);
 const char* dsh_server_last_error(dsh_server* sv);
 void dsh_server_destroy(dsh_server* sv);
 

@@ -1,6 +1,7 @@
 """Scheduling logic of the continuous-batching server on the host-only fake device:
 arrival-order admission under the HBM budget, rejection, cancellation, recovery from a
-failed rebuild, and the time accounting the p50/p99 report uses (injected clock)."""
+failed rebuild, finished tubes that wait for the next admission instead of forcing a
+rebuild, and the time accounting the p50/p99 report uses (injected clock)."""
 import pytest
 
 from dustyshock3d_amd import abi, configs
@@ -104,3 +105,30 @@ def test_new_arrivals_join_a_running_batch():
     pa, pb = sv.poll(a), sv.poll(b)
     assert pb["finished"] < pa["finished"]
     assert pytest.approx(pb["particle_steps"]) == 20 * 2 * 900
+
+
+def test_finished_tubes_cost_no_rebuild_of_their_own():
+    """A tube that finishes stays in the batch (skipped by the kernels) until requests are
+    admitted; the batch is released once nothing runs."""
+    fp = footprint(small(0))
+    sv = Server(hbm_budget=8 * fp, hbm_capacity=16 * fp, steps_per_epoch=10, backend=abi.BACKEND_FAKE,
+                clock=Clock())
+    ids = [sv.submit(small(i, steps), 0.0) for i, steps in enumerate((20, 50, 90))]
+    run_until_idle(sv)
+    assert [sv.poll(i)["state"] for i in ids] == [abi.REQ_DONE] * 3
+    assert sv.stats()["rebuilds"] == 2            # admission + release
+    assert sv.stats()["bytes_in_use"] == 0
+    fin = [sv.poll(i)["finished"] for i in ids]
+    assert fin == sorted(fin) and len(set(fin)) == 3
+    # an admission drops the finished tubes and keeps the running one
+    a = sv.submit(small(3, 100), 0.0)
+    sv.pump()
+    b = sv.submit(small(4, 10), 0.0)
+    sv.pump()
+    sv.pump()
+    assert sv.poll(b)["state"] == abi.REQ_DONE and sv.poll(a)["state"] == abi.REQ_RUNNING
+    c = sv.submit(small(5, 10), 0.0)
+    sv.pump()                                      # rebuild: a kept, b dropped, c added
+    assert sv.stats()["bytes_in_use"] == pytest.approx(2 * fp)
+    run_until_idle(sv)
+    assert sv.poll(a)["state"] == abi.REQ_DONE and sv.poll(c)["state"] == abi.REQ_DONE
