@@ -1703,8 +1703,10 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
 // One wave per run: lanes take 64 particles at a time; their contributions go to LDS and lane
 // j sums cell j's in sorted order, solves cell j and leaves its drag terms in LDS for the
 // particles of the cell.  One conservation / CFL partial per run.
+// The kernel is latency-bound (a chain of dependent loads, sums and FP64 divisions per wave),
+// so registers are capped for 8 waves per SIMD: more waves in flight beat the few spills.
 constexpr int kRunCells = 8;
-__global__ __launch_bounds__(kBS) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
                                                    const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                                    const int* __restrict__ start, const int* __restrict__ ginv,
                                                    double* __restrict__ part) {
