@@ -30,14 +30,14 @@ HBM_PEAK_GBS = 8000.0
 # (gas, dust).  The idic stage fuses the cell reduction (60 / 52) with the drag +
 # kick-drift + energy + conservation pass (108 / 96).  Per averaging cell of one neighbour
 # cell it adds the cell start of each species (2 x 4 B) and a 40 B conservation / CFL
-# partial per run of 8 cells (5 B): 13 B.
+# partial per block of 32 cells (1.25 B): 9 B.
 ALGO_BYTES = {
     "sort": (144, 136),
     "density": (28, 28),
     "force": (68, 0),
     "idic": (168, 148),
 }
-AVG_CELL_BYTES = 13
+AVG_CELL_BYTES = 9
 SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "radius")
 
 

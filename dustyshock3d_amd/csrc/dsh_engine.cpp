@@ -262,7 +262,7 @@ int Engine::prepare() {
     for (int t = 0; t < T; ++t) {
         TubeConst& C = tc_h_[t];
         C.avg_off = (int)aoff;
-        C.npart = C.f == 1 ? (C.navg + idic_run_cells() - 1) / idic_run_cells() : C.navg;
+        C.npart = C.f == 1 ? (C.navg + idic_block_cells() - 1) / idic_block_cells() : C.navg;
         (C.f == 1 ? a_.idic_runs : a_.idic_blocks) = true;
         aseg_.push_back(Seg{(int)aoff, C.navg, t, 0});
         aoff += C.navg;
@@ -297,7 +297,7 @@ int Engine::prepare() {
     if ((rc = make_map(gseg_, 256, a_.gmap))) return rc;
     if ((rc = make_map(cseg_, 256, a_.cmap))) return rc;
     if ((rc = make_map(aseg_, 4, a_.amap))) return rc;
-    if ((rc = make_map(aseg_, 4 * idic_run_cells(), a_.amap_runs))) return rc;
+    if ((rc = make_map(aseg_, idic_block_cells(), a_.amap_runs))) return rc;
     if ((rc = make_map(pseg_, 64, a_.pmap64))) return rc;
     if ((rc = make_map(gseg_, 64, a_.gmap64))) return rc;
     a_.variant = pair_variant_;

@@ -53,7 +53,7 @@ struct StepArgs {
     // launch maps
     Map pmap, gmap, cmap, amap;
     Map pmap64, gmap64;    // 64 targets per block (tiled pair kernels)
-    Map amap_runs;         // runs of idic_run_cells() averaging cells per wave (tubes with f == 1)
+    Map amap_runs;         // idic_block_cells() averaging cells per block, idic_run_cells() per wave (f == 1)
     bool idic_blocks, idic_runs;   // which drag kernels the batch needs
     unsigned long long* masks;   // gas density hit masks reused by the force pass
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
@@ -72,7 +72,8 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s);
 hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s);
 hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s);
 hipError_t launch_idic(const StepArgs& a, hipStream_t s);
-int idic_run_cells();   // averaging cells per wave of the f == 1 drag kernel
+int idic_run_cells();     // averaging cells per wave of the f == 1 drag kernel
+int idic_block_cells();   // averaging cells per block of it (one conservation partial each)
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s);
 hipError_t launch_flip(const StepArgs& a, hipStream_t s);
 // seeded shock-tube set-up of one (tube, species) straight into buffer slots off..off+n-1

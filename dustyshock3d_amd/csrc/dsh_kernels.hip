@@ -1702,7 +1702,7 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
 // run of kRunCells consecutive cells is one contiguous range of sorted particles per species.
 // One wave per run: lanes take 64 particles at a time; their contributions go to LDS and lane
 // j sums cell j's in sorted order, solves cell j and leaves its drag terms in LDS for the
-// particles of the cell.  One conservation / CFL partial per run.
+// particles of the cell.  One conservation / CFL partial per block (its runs in order).
 // The kernel is latency-bound (a chain of dependent loads, sums and FP64 divisions per wave),
 // so registers are capped for 8 waves per SIMD: more waves in flight beat the few spills.
 constexpr int kRunCells = 8;
@@ -1712,90 +1712,102 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
                                                    double* __restrict__ part) {
     __shared__ double sv[kBS / 64][6][64];
     __shared__ CellDrag cd[kBS / 64][kRunCells];
+    __shared__ double bp[kBS / 64][5];
     const ItemPos ip = locate(mp, (kBS / 64) * kRunCells);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int la0 = ip.li + wave * kRunCells;
-    if (la0 >= ip.sg.n) return;
     const int t = ip.sg.tube;
     const TubeDyn& D = td[t];
-    if (D.done || D.err) return;
+    if (D.done || D.err) return;   // the block's cells are all of tube t
     const TubeConst& T = tc[t];
     if (T.f != 1) return;   // k_idic
-    const long long wo = (long long)(1 - *cur) * B.ntot;
-    const int ncl = min(kRunCells, ip.sg.n - la0);
-    const int nd = T.nsp - 1;
-    const double tau = D.dt;
-    double (*V)[64] = sv[wave];
+    double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
+    if (la0 < ip.sg.n) {
+        const long long wo = (long long)(1 - *cur) * B.ntot;
+        const int ncl = min(kRunCells, ip.sg.n - la0);
+        const int nd = T.nsp - 1;
+        const double tau = D.dt;
+        double (*V)[64] = sv[wave];
 
-    // lane j < ncl: sums of cell la0 + j
-    CellSums S{};
+        // lane j < ncl: sums of cell la0 + j
+        CellSums S{};
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) {
-        if (s >= T.nsp) break;
-        const int c0 = T.cell_off[s] + la0;
-        const int p0 = start[c0], p1 = start[c0 + ncl];
-        const int jlo = lane < ncl ? start[c0 + lane] : 0, jhi = lane < ncl ? start[c0 + lane + 1] : 0;
-        if (lane < ncl) {
-            if (s == 0) S.ng = jhi - jlo;
-            else S.ndc[s - 1] = jhi - jlo;
-        }
-        for (int base = p0; base < p1; base += 64) {
-            const int k = base + lane;
-            if (k < p1) {
-                const long long g = wo + k;
-                V[0][lane] = B.vx[g];
-                V[1][lane] = B.vy[g];
-                V[2][lane] = B.vz[g];
-                if (s == 0) {
-                    V[3][lane] = B.ax[g];
-                    V[4][lane] = B.ay[g];
-                    V[5][lane] = B.az[g];
-                } else {
-                    V[3][lane] = dust_tstop(T, B, ginv, wo, k, s);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+        for (int s = 0; s < kMaxSpecies; ++s) {
+            if (s >= T.nsp) break;
+            const int c0 = T.cell_off[s] + la0;
+            const int p0 = start[c0], p1 = start[c0 + ncl];
+            const int jlo = lane < ncl ? start[c0 + lane] : 0, jhi = lane < ncl ? start[c0 + lane + 1] : 0;
             if (lane < ncl) {
-                const int a = max(jlo, base) - base, e = min(jhi, base + 64) - base;
-                for (int u = a; u < e; ++u) {
+                if (s == 0) S.ng = jhi - jlo;
+                else S.ndc[s - 1] = jhi - jlo;
+            }
+            for (int base = p0; base < p1; base += 64) {
+                const int k = base + lane;
+                if (k < p1) {
+                    const long long g = wo + k;
+                    V[0][lane] = B.vx[g];
+                    V[1][lane] = B.vy[g];
+                    V[2][lane] = B.vz[g];
                     if (s == 0) {
-                        for (int c = 0; c < 6; ++c) S.sg[c] += V[c][u];
+                        V[3][lane] = B.ax[g];
+                        V[4][lane] = B.ay[g];
+                        V[5][lane] = B.az[g];
                     } else {
-                        for (int c = 0; c < 4; ++c) S.sd[s - 1][c] += V[c][u];
+                        V[3][lane] = dust_tstop(T, B, ginv, wo, k, s);
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
+                if (lane < ncl) {
+                    const int a = max(jlo, base) - base, e = min(jhi, base + 64) - base;
+                    for (int u = a; u < e; ++u) {
+                        if (s == 0) {
+                            for (int c = 0; c < 6; ++c) S.sg[c] += V[c][u];
+                        } else {
+                            for (int c = 0; c < 4; ++c) S.sd[s - 1][c] += V[c][u];
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
-    }
-    if (lane < ncl) cd[wave][lane] = cell_drag(T, nd, tau, S);
-    __builtin_amdgcn_wave_barrier();
+        if (lane < ncl) cd[wave][lane] = cell_drag(T, nd, tau, S);
+        __builtin_amdgcn_wave_barrier();
 
-    double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) {
-        if (s >= T.nsp) break;
-        const int c0 = T.cell_off[s] + la0;
-        const int p0 = start[c0], p1 = start[c0 + ncl];
-        for (int k = p0 + lane; k < p1; k += 64) {
-            // the cell of k: the last cell of the run whose start is <= k
-            int j = 0;
-            for (int jj = 1; jj < ncl; ++jj) j = (start[c0 + jj] <= k) ? jj : j;
-            particle_update(T, B, wo + k, s, nd, tau, cd[wave][j], px, py, pz, E, mn);
+        for (int s = 0; s < kMaxSpecies; ++s) {
+            if (s >= T.nsp) break;
+            const int c0 = T.cell_off[s] + la0;
+            const int p0 = start[c0], p1 = start[c0 + ncl];
+            for (int k = p0 + lane; k < p1; k += 64) {
+                // the cell of k: the last cell of the run whose start is <= k
+                int j = 0;
+                for (int jj = 1; jj < ncl; ++jj) j = (start[c0 + jj] <= k) ? jj : j;
+                particle_update(T, B, wo + k, s, nd, tau, cd[wave][j], px, py, pz, E, mn);
+            }
         }
+        px = wave_sum(px);
+        py = wave_sum(py);
+        pz = wave_sum(pz);
+        E = wave_sum(E);
+        mn = wave_min(mn);
     }
-    px = wave_sum(px);
-    py = wave_sum(py);
-    pz = wave_sum(pz);
-    E = wave_sum(E);
-    mn = wave_min(mn);
     if (lane == 0) {
-        double* o = part + 5ll * (T.avg_off + la0 / kRunCells);
-        o[0] = px;
-        o[1] = py;
-        o[2] = pz;
-        o[3] = E;
-        o[4] = mn;
+        bp[wave][0] = px;
+        bp[wave][1] = py;
+        bp[wave][2] = pz;
+        bp[wave][3] = E;
+        bp[wave][4] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q[4] = {0, 0, 0, 0}, m2 = 1e300;
+        for (int w = 0; w < kBS / 64; ++w) {
+            for (int c = 0; c < 4; ++c) q[c] += bp[w][c];
+            m2 = fmin(m2, bp[w][4]);
+        }
+        double* o = part + 5ll * (T.avg_off + ip.li / ((kBS / 64) * kRunCells));
+        for (int c = 0; c < 4; ++c) o[c] = q[c];
+        o[4] = m2;
     }
 }
 
@@ -2035,6 +2047,7 @@ hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
 }
 
 int idic_run_cells() { return kRunCells; }
+int idic_block_cells() { return (kBS / 64) * kRunCells; }
 
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_finalize, dim3(a.ntubes), dim3(kFinBS), 0, s, a.tc, a.td, a.cur, a.part);
