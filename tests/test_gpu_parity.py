@@ -132,3 +132,35 @@ def test_cfg0_geometry_mixed_run(oracle_mod):
     e_p = momentum_error(g.conservation(), o.conservation(), st_o)
     print(f"momentum {e_p:.3e}")
     assert e_p <= 1e-7
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+@pytest.mark.parametrize("avg_factor", [2, 3])
+def test_wide_averaging_cells_parity(oracle_mod, precision, avg_factor):
+    """Averaging cells of f x f x f neighbour cells (f >= 2, one wave per averaging cell):
+    10 steps of the cfg0 tube against the oracle, which restates the same grouping."""
+    spec = configs.cfg0(max_steps=10)
+    o0 = oracle_mod.OracleSim.from_spec(spec)
+    params, state = o0.params, take_state(o0)
+    params.avg_factor = avg_factor
+    params.precision = precision
+    print(f"\ncase avg_factor={avg_factor} precision={precision}", flush=True)
+    o = inject(oracle_mod.OracleSim, params, state)
+    g = inject(Simulation, params, state)
+    o.step(10)
+    g.step(10)
+    so, sg = o.status(), g.status()
+    assert sg["steps"] == so["steps"] == 10
+    st_o = take_state(o)
+    tol = 1e-9 if precision == FP64 else 1e-5
+    for s in (0, 1):
+        ro, rg = st_o[s], g.download(s)
+        assert np.array_equal(rg["cell"], ro["cell"])
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"species {s}: rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= tol
+        assert e_v <= tol
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    assert e_p <= (1e-9 if precision == FP64 else 1e-7)
