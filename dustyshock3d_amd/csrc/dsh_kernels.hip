@@ -881,16 +881,15 @@ __device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double y
     return dd < hs * hs ? 1u : 0u;
 }
 
-// The density pass pads the position arrays by one window: a window reads base + t, t < 64,
-// with base clamped to the staged count, and bits past a lane's own range are masked.  The
-// force pass reads the density pass's hit masks; its rare fallback (more windows than mask
-// slots) stays inside each lane's range, so it needs no padding.  Separate arrays (not one
+// A density window reads slots [base, base + wn4) with base = min(a, kTileP - wn4), inside the
+// piece for every lane, and shifts the lane's bits down by a - base; bits past a lane's own
+// range are masked.  The force pass reads the density pass's hit masks; its rare fallback
+// (more windows than mask slots) stays inside each lane's range.  Separate arrays (not one
 // record per candidate) keep the lanes' 8-byte reads on distinct banks.  Velocities are
 // staged in the pair-arithmetic type R: v_i - v_j is formed in R, antisymmetric bit for bit.
 template <typename R, int KIND, bool kStats, int kTileP>
 struct TileLds {
-    static constexpr int kPad = KIND == 0 ? 64 : 0;
-    double x[kTileP + kPad], y[kTileP + kPad], z[kTileP + kPad], h[kTileP + kPad];
+    double x[kTileP], y[kTileP], z[kTileP], h[kTileP];
     R vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
     R m[KIND ? kTileP : 1];
     R mw[KIND ? 1 : kTileP];    // density: m / pi
@@ -942,7 +941,7 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
 // the same iteration order and reads them instead of re-running the predicate.
 // Occupancy targets: both passes are latency- as well as VALU-bound, so the register
 // budget is capped at 4 (force) / 5 (density) waves per SIMD; with pieces of 160 candidates
-// the LDS of a wave (10 KB force, 8.4 KB density) allows as many.
+// the LDS of a wave (10 KB force, 6.4 KB density) allows as many.
 template <typename R, int KIND, bool kStats, int kTileP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const int* __restrict__ cur,
@@ -952,6 +951,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                                                     int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
                                                     unsigned long long* __restrict__ masks, int mask_chunks) {
     static_assert(kTileP <= 256, "window lengths are reduced with wave_max_u9");
+    static_assert(kTileP >= 64, "a density window of 64 slots fits in one piece");
     __shared__ TileLds<R, KIND, kStats, kTileP> L;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int sgi = find_seg(mp, b);
@@ -1061,8 +1061,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     };
     // Hit mask of one window of 64 candidates of this lane's range, starting at staged slot
     // a (>= 0); len = candidates of the lane's range from a on, wn = the wave's maximum of
-    // len (<= 64 used), pn = staged count.  The force pass reads the density pass's mask.
-    auto window_mask = [&](int a, int len, int wn, int pn, const RowSeg& g) -> unsigned long long {
+    // len (<= 64 used).  The force pass reads the density pass's mask.
+    auto window_mask = [&](int a, int len, int wn, const RowSeg& g) -> unsigned long long {
         unsigned long long mask = 0ull;
         if (KIND == 1 && use_masks && slot < kMaskSlots) {
             mask = mrow[(size_t)slot * 64];
@@ -1077,14 +1077,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 mask |= (unsigned long long)hit << t;
             }
         } else {
-            // uniform trip count: every lane evaluates wn candidates from a base clamped into
-            // the padded piece; bits past the lane's own range are cleared once per window
+            // uniform trip count: every lane evaluates wn4 slots from a base that keeps them
+            // inside the piece (a lane whose window starts past base finds candidate a + u on
+            // bit a - base + u); bits past the lane's own range are cleared once per window
             wn = min(wn, 64);
             const unsigned long long valid = len >= 64 ? ~0ull : (len <= 0 ? 0ull : ((1ull << len) - 1ull));
-            const int base = min(a, pn);
+            const int base = min(a, kTileP - ((wn + 3) & ~3));
+            const int shift = a - base;
             // candidates are shifted in from the top, so candidate t lands on bit t
             const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
-            // whole groups of 4 (reads past wn stay in the padding; their bits are masked)
+            // whole groups of 4 (slots past a lane's window are masked)
             const int wn4 = (wn + 3) & ~3;
             const int n0 = min(wn4, 32);
             uint32_t mlo = 0u, mhi = 0u;
@@ -1105,7 +1107,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 for (int t = wn4 - 4; t >= 32; t -= 4) mhi = group(std::true_type{}, mhi, t);
                 for (int t = n0 - 4; t >= 0; t -= 4) mlo = group(std::true_type{}, mlo, t);
             }
-            mask = (((unsigned long long)mhi << 32) | mlo) & valid;
+            const unsigned long long raw = ((unsigned long long)mhi << 32) | mlo;
+            mask = shift >= 64 ? 0ull : (raw >> shift) & valid;
         }
         if (KIND == 0 && use_masks && slot < kMaskSlots) mrow[(size_t)slot * 64] = mask;
         ++slot;
@@ -1254,7 +1257,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
         for (int ip = 0; ip < npieces; ++ip) {
             const int pb = A.ub + ip * kTileP;
             const int pnA = pair ? nA : min(kTileP, A.ue - pb);
-            const int pn = pnA + nB;
             stage(pb, pnA, B2.ub, nB);
             int lenA, lenB = 0;
             const int aA = seg_window_range(max(A.jl, pb) - pb, min(A.jh, pb + pnA) - pb, lenA);
@@ -1276,7 +1278,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                     if (!use) return;
                     const int o = (second ? aB : aA) + 64 * w;
                     const unsigned long long m =
-                        window_mask(o, (second ? lenB : lenA) - 64 * w, (second ? mlB : mlA) - 64 * w, pn,
+                        window_mask(o, (second ? lenB : lenA) - 64 * w, (second ? mlB : mlA) - 64 * w,
                                     second ? B2 : A);
                     if (second) {
                         mB = m;
