@@ -38,6 +38,7 @@ ALGO_BYTES = {
     "idic": (168, 148),
 }
 AVG_CELL_BYTES = 9
+DEFAULT_TILE = 192   # the library's default piece size of the tiled pair kernels
 SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "radius")
 
 
@@ -55,7 +56,7 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
 
 
 # kernels behind each pass, as named in the rocprofv3 summaries under profiles/
-def pass_kernels(stage, tile=160):
+def pass_kernels(stage, tile=DEFAULT_TILE):
     """Kernel names (as rocprofv3 reports them) of one pass of the step."""
     return {
         "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
@@ -66,7 +67,7 @@ def pass_kernels(stage, tile=160):
     }.get(stage, [])
 
 
-def measured_traffic(workload, stage, tile=160):
+def measured_traffic(workload, stage, tile=DEFAULT_TILE):
     """HBM bytes per launch of a pass from the committed FETCH_SIZE/WRITE_SIZE summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not os.path.exists(path):
@@ -78,7 +79,7 @@ def measured_traffic(workload, stage, tile=160):
     return sum(kern[n]["read_bytes"] + kern[n]["write_bytes"] for n in names)
 
 
-def measured_valu(workload, stage, tile=160):
+def measured_valu(workload, stage, tile=DEFAULT_TILE):
     """VALU issue fraction of a pass's kernel from the committed SQ-counter summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
     if not os.path.exists(path):
@@ -117,7 +118,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default piece size (160)")
+    ap.add_argument("--no-autotune", action="store_true", help=f"keep the default piece size ({DEFAULT_TILE})")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -191,12 +192,12 @@ def main():
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else 160)
+                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else DEFAULT_TILE)
                      if args.variant == 1 else None,
                      "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms,
                      # the pair passes are compute-bound: the share of VALU issue slots they use
-                     "valu_issue_frac": measured_valu(args.config, dominant, tune["tile"] if tune else 160)
+                     "valu_issue_frac": measured_valu(args.config, dominant, tune["tile"] if tune else DEFAULT_TILE)
                      if args.variant == 1 else None,
                      "valu_source": f"profiles/valu_{args.config}.json (rocprofv3 SQ_INSTS_VALU)"},
         "step_roofline": {"algo_bytes": step_bytes, "event_ms": step_ms,

@@ -109,7 +109,7 @@ private:
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
     int precision_ = 0;
     int pair_variant_ = kPairTiled;
-    int tile_ = 160;
+    int tile_ = 192;
     long long ntot_ = 0;
     size_t dev_bytes_ = 0;
     std::vector<std::pair<std::string, double>> ktimes_;

@@ -881,15 +881,35 @@ __device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double y
     return dd < hs * hs ? 1u : 0u;
 }
 
+// 32-bit fixed point of a periodic-box coordinate: units of L / 2^29 from the box origin,
+// so every coordinate within four box lengths fits and an image shift is 2^29 units.
+// Differences are taken modulo 2^32, exact for any pair closer than 4 L.
+constexpr double kFixUnits = 536870912.0;   // 2^29
+__device__ __forceinline__ int fix_coord(double v, double o, double units_per_length) {
+    return __double2int_rn((v - o) * units_per_length);
+}
+__device__ __forceinline__ int fix_diff(int a, int b, int images) {
+    return (int)((unsigned)a - (unsigned)b + (unsigned)images * 536870912u);
+}
+
 // A density window reads slots [base, base + wn4) with base = min(a, kTileP - wn4), inside the
 // piece for every lane, and shifts the lane's bits down by a - base; bits past a lane's own
 // range are masked.  The force pass reads the density pass's hit masks; its rare fallback
 // (more windows than mask slots) stays inside each lane's range.  Separate arrays (not one
 // record per candidate) keep the lanes' 8-byte reads on distinct banks.  Velocities are
 // staged in the pair-arithmetic type R: v_i - v_j is formed in R, antisymmetric bit for bit.
+// The FP32 force pass stages y and z as 32-bit fixed point and h as FP32 (x stays FP64):
+// an integer difference negates exactly, so dy and dz stay antisymmetric, and the smaller
+// record lets larger pieces (more mirror-row pairs) keep 4 waves per SIMD.
+template <typename R, int KIND>
+constexpr bool kFixYZ = KIND == 1 && sizeof(R) == 4;
 template <typename R, int KIND, bool kStats, int kTileP>
 struct TileLds {
-    double x[kTileP], y[kTileP], z[kTileP], h[kTileP];
+    using YZ = std::conditional_t<kFixYZ<R, KIND>, int, double>;
+    using HT = std::conditional_t<kFixYZ<R, KIND>, float, double>;
+    double x[kTileP];
+    YZ y[kTileP], z[kTileP];
+    HT h[kTileP];
     R vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
     R m[KIND ? kTileP : 1];
     R mw[KIND ? 1 : kTileP];    // density: m / pi
@@ -1007,6 +1027,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const R alpha = (R)T.alpha, beta = (R)T.beta, epsv = (R)T.eps;
     const bool plain = T.nsp == 1;   // gas-only run: de/dt of the plain ForceKernel
     const R hhi = (R)(0.5 * hi);
+    constexpr bool kFix = kFixYZ<R, KIND>;
+    const double ups_y = kFixUnits / T.L[1], ups_z = kFixUnits / T.L[2];
+    const float qy = (float)(T.L[1] / kFixUnits), qz = (float)(T.L[2] / kFixUnits);
+    const int Yi = fix_coord(yi, T.o[1], ups_y), Zi = fix_coord(zi, T.o[2], ups_z);
+    const float hif = (float)hi;
 
     // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
     // range [ub, ue) of all lanes, this lane's own range [jl, jh) (empty if it does not need
@@ -1039,9 +1064,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             if (kStats) id = B.id[g];
             __builtin_amdgcn_sched_barrier(0);
             L.x[k] = x;
-            L.y[k] = y;
-            L.z[k] = z;
-            L.h[k] = h;
+            if constexpr (kFix) {
+                L.y[k] = fix_coord(y, T.o[1], ups_y);
+                L.z[k] = fix_coord(z, T.o[2], ups_z);
+                L.h[k] = (float)h;
+            } else {
+                L.y[k] = y;
+                L.z[k] = z;
+                L.h[k] = h;
+            }
             if (KIND == 0) {
                 L.mw[k] = (R)(m * kInvPi);
                 L.hh[k] = (R)(0.5 * h);
@@ -1062,19 +1093,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     // Hit mask of one window of 64 candidates of this lane's range, starting at staged slot
     // a (>= 0); len = candidates of the lane's range from a on, wn = the wave's maximum of
     // len (<= 64 used).  The force pass reads the density pass's mask.
-    auto window_mask = [&](int a, int len, int wn, const RowSeg& g) -> unsigned long long {
+    auto window_mask = [&](int a, int len, int wn, const RowSeg& g, int gb) -> unsigned long long {
         unsigned long long mask = 0ull;
-        if (KIND == 1 && use_masks && slot < kMaskSlots) {
-            mask = mrow[(size_t)slot * 64];
-        } else if (KIND == 1) {
-            // no stored mask: each lane tests its own candidates only
-            const int nt = min(max(len, 0), 64);
-            for (int t = 0; t < nt; ++t) {
-                const int u = a + t;
-                const P4 c{L.x[u], L.y[u], L.z[u], L.h[u]};
-                const uint32_t hit = g.shifted ? pred1_shift(c, xi, yi, zi, hi, g.shy, g.shz)
-                                               : pred1(c, xi, yi, zi, hi);
-                mask |= (unsigned long long)hit << t;
+        if constexpr (KIND == 1) {
+            if (use_masks && slot < kMaskSlots) {
+                mask = mrow[(size_t)slot * 64];
+            } else {
+                // no stored mask: each lane tests its own candidates only
+                const int nt = min(max(len, 0), 64);
+                for (int t = 0; t < nt; ++t) {
+                    const int u = a + t;
+                    P4 c;
+                    double ty = yi, tz = zi, th = hi;
+                    if constexpr (kFix) {
+                        // FP64 y, z, h of the candidate (sorted index gb + u) and of the target
+                        // from memory
+                        const long long gg = wo + gb + u;
+                        c = P4{L.x[u], B.y[gg], B.z[gg], B.h[gg]};
+                        int qq = q;
+                        asm volatile("" : "+v"(qq));
+                        ty = B.y[wo + qq];
+                        tz = B.z[wo + qq];
+                        th = B.h[wo + qq];
+                    } else {
+                        c = P4{L.x[u], (double)L.y[u], (double)L.z[u], (double)L.h[u]};
+                    }
+                    const uint32_t hit = g.shifted ? pred1_shift(c, xi, ty, tz, th, g.shy, g.shz)
+                                                   : pred1(c, xi, ty, tz, th);
+                    mask |= (unsigned long long)hit << t;
+                }
             }
         } else {
             // uniform trip count: every lane evaluates wn4 slots from a base that keeps them
@@ -1118,6 +1165,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     auto hit = [&](auto shifted_tag, int u, double shy, double shz, int sy, int sz, int gi) {
         constexpr bool kSh = decltype(shifted_tag)::value;
         const double dx = L.x[u] - xi;
+        if constexpr (kFix) {
+            if (L.ind[u] != 0) {
+                // a frozen neighbour has no partner term: FP64 keeps the momentum it injects
+                // exact to rounding (its y, z, h and the target's values are read from memory;
+                // the empty asm keeps their addresses from being hoisted out of the loops)
+                int qq = q;
+                asm volatile("" : "+v"(qq));
+                const long long g = wo + gi, gq = wo + qq;
+                const double dy = kSh ? (B.y[g] - B.y[gq]) + shy : B.y[g] - B.y[gq];
+                const double dz = kSh ? (B.z[g] - B.z[gq]) + shz : B.z[g] - B.z[gq];
+                const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                if (dd > 0.0) {
+                    double coef, edc;
+                    force_terms<double>(dd, 0.5 * (B.h[gq] + B.h[g]), dx, dy, dz, B.vx[gq] - B.vx[g],
+                                        B.vy[gq] - B.vy[g], B.vz[gq] - B.vz[g], B.rho[gq], B.rho[g], pr2[qq],
+                                        pr2[gi], csav[qq], csav[gi], B.m[gq], B.m[g], T.alpha, T.beta, T.eps,
+                                        plain, coef, edc);
+                    d0 += dx * coef;
+                    d1 += dy * coef;
+                    d2s += dz * coef;
+                    d3 += edc;
+                }
+            } else {
+                // dx from FP64 positions, dy and dz from fixed-point differences (the periodic
+                // image is sy, sz box lengths); every term is antisymmetric in (i, j)
+                const float fx = (float)dx;
+                const float fy = (float)fix_diff(L.y[u], Yi, sy) * qy;
+                const float fz = (float)fix_diff(L.z[u], Zi, sz) * qz;
+                const float dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+                if (dd > 0.0f) {
+                    R coef, edc;
+                    force_terms<R>(dd, 0.5f * (hif + L.h[u]), fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
+                                   vzi_r - L.vz[u], rhoi, L.rho[u], pr2i, L.pr2[u], csi, L.cs[u], mi, L.m[u],
+                                   alpha, beta, epsv, plain, coef, edc);
+                    a0.add(fx * coef);
+                    a1.add(fy * coef);
+                    a2.add(fz * coef);
+                    a3.add(edc);
+                }
+            }
+            return;
+        }
         const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
         const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
         const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
@@ -1279,7 +1368,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                     const int o = (second ? aB : aA) + 64 * w;
                     const unsigned long long m =
                         window_mask(o, (second ? lenB : lenA) - 64 * w, (second ? mlB : mlA) - 64 * w,
-                                    second ? B2 : A);
+                                    second ? B2 : A, second ? B2.ub - nA : pb);
                     if (second) {
                         mB = m;
                         oB = o;

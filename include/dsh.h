@@ -178,7 +178,7 @@ int dsh_set_stats(dsh_ctx* ctx, int on);
 #define DSH_PAIR_DIRECT 0
 #define DSH_PAIR_TILED 1
 int dsh_set_pair_variant(dsh_ctx* ctx, int variant);
-/* Candidates staged per LDS piece by the tiled pair kernels: 128, 160 (default), 192, 224 or 256.
+/* Candidates staged per LDS piece by the tiled pair kernels: 128, 160, 192 (default), 224 or 256.
    Pieces split each candidate row in order, so every size gives the same results bit for bit. */
 int dsh_set_tile(dsh_ctx* ctx, int tile);
 /* Autotuner (no reference counterpart): runs steps_per_trial steps with each piece size
