@@ -1032,6 +1032,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const float qy = (float)(T.L[1] / kFixUnits), qz = (float)(T.L[2] / kFixUnits);
     const int Yi = fix_coord(yi, T.o[1], ups_y), Zi = fix_coord(zi, T.o[2], ups_z);
     const float hif = (float)hi;
+    // a neighbour within reach more than ~4 box lengths away across a periodic image would
+    // overflow the fixed-point difference: such waves take dy, dz from FP64 positions instead
+    const bool far_images = kFix && __ballot(rch >= 3.5 * fmin(T.L[1], T.L[2])) != 0ull;
 
     // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
     // range [ub, ue) of all lanes, this lane's own range [jl, jh) (empty if it does not need
@@ -1191,9 +1194,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 // dx from FP64 positions, dy and dz from fixed-point differences (the periodic
                 // image is sy, sz box lengths); every term is antisymmetric in (i, j)
                 const float fx = (float)dx;
-                const float fy = (float)fix_diff(L.y[u], Yi, sy) * qy;
-                const float fz = (float)fix_diff(L.z[u], Zi, sz) * qz;
-                const float dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+                float fy, fz, dd;
+                if (!far_images) {
+                    fy = (float)fix_diff(L.y[u], Yi, sy) * qy;
+                    fz = (float)fix_diff(L.z[u], Zi, sz) * qz;
+                    dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+                } else {
+                    int qq = q;
+                    asm volatile("" : "+v"(qq));
+                    const long long g = wo + gi, gq = wo + qq;
+                    const double dy = kSh ? (B.y[g] - B.y[gq]) + shy : B.y[g] - B.y[gq];
+                    const double dz = kSh ? (B.z[g] - B.z[gq]) + shz : B.z[g] - B.z[gq];
+                    fy = (float)dy;
+                    fz = (float)dz;
+                    dd = (float)__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                }
                 if (dd > 0.0f) {
                     R coef, edc;
                     force_terms<R>(dd, 0.5f * (hif + L.h[u]), fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],

@@ -164,3 +164,33 @@ def test_wide_averaging_cells_parity(oracle_mod, precision, avg_factor):
         assert e_v <= tol
     e_p = momentum_error(g.conservation(), o.conservation(), st_o)
     assert e_p <= (1e-9 if precision == FP64 else 1e-7)
+
+
+def test_neighbours_many_periodic_images_away(oracle_mod):
+    """A tube 0.001 wide with h = 0.006 in the shared state: neighbours sit up to 12 periodic
+    images away in y and z, beyond the range of the FP32 force pass's fixed-point y, z
+    differences (4 box lengths), so those waves take dy, dz from FP64 positions.  One step in
+    mixed precision: sets bit-exact, fields within the mixed tolerances."""
+    width, h = 0.001, 0.006
+    spec = configs.tube(width, 0.01, 10.0, 1.0, max_steps=1, precision=MIXED)
+    o0 = oracle_mod.OracleSim.from_spec(spec)
+    params, state = o0.params, take_state(o0)
+    for s in (0, 1):
+        state[s]["h"] = np.full_like(state[s]["h"], h)
+    assert 2 * h >= 4 * width
+    o, g = _one_step_both(oracle_mod, params, state, MIXED)
+    for s in (0, 1):
+        ro, rg = o.download(s), g.download(s)
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co) and np.array_equal(hg, ho)
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"\nimages species {s}: neighbours {float(cg.mean()):.0f} per particle, rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= 1e-5
+        assert e_v <= 1e-5
+        if s == 0:
+            e_a = vec_normwise(rg, ro, ("ax", "ay", "az"))
+            print(f"gas A {e_a:.3e}")
+            assert e_a <= 1e-5
