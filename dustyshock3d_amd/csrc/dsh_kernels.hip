@@ -960,8 +960,8 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
 // the same iteration order and reads them instead of re-running the predicate.
 // Occupancy targets: both passes are latency- as well as VALU-bound, so the register
-// budget is capped at 4 (force) / 5 (density) waves per SIMD; with pieces of 160 candidates
-// the LDS of a wave (10 KB force, 6.4 KB density) allows as many.
+// budget is capped at 4 (force) / 5 (density) waves per SIMD; with pieces of 192 candidates
+// the LDS of a wave (9.4 KB FP32 force, 7.7 KB FP32 density) allows as many.
 template <typename R, int KIND, bool kStats, int kTileP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const int* __restrict__ cur,
