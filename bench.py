@@ -91,8 +91,7 @@ def measured_valu(workload, stage, tile=DEFAULT_TILE):
     return kern[names[0]]["valu_issue_frac"]
 
 
-def cpu_baseline(spec, threads, steps=1):
-    """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube."""
+def _oracle_rate(spec, threads, steps):
     import oracle
 
     oracle.set_threads(threads)
@@ -102,10 +101,32 @@ def cpu_baseline(spec, threads, steps=1):
     o.step(steps)
     dt = time.perf_counter() - t0
     n = sum(o.counts.values())
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": oracle.get_threads(),
-            "kind": "port",
+    return n * steps / dt, n, dt, oracle.get_threads()
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(spec, threads, steps=1):
+    """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube; and on one
+    thread, one step of the cfg1-sized tube (a quarter of the particles)."""
+    from dustyshock3d_amd import configs
+
+    rate, n, dt, cores = _oracle_rate(spec, threads, steps)
+    small = configs.cfg1()
+    rate1, n1, dt1, _ = _oracle_rate(small, 1, 1)
+    return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"{steps} step(s) of the full {n}-particle tube (first step from the initial state), "
-                      f"FP64 OpenMP oracle, {dt:.1f} s"}
+                      f"FP64 OpenMP oracle, {dt:.1f} s",
+            "one_thread": {"value": rate1, "sample": f"1 step of the {n1}-particle cfg1 tube, {dt1:.1f} s"},
+            "cpu_model": _cpu_model()}
 
 
 def main():
