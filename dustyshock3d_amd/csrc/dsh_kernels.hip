@@ -881,15 +881,15 @@ __device__ __forceinline__ uint32_t pred1_shift(const P4& c, double xi, double y
     return dd < hs * hs ? 1u : 0u;
 }
 
-// 32-bit fixed point of a periodic-box coordinate: units of L / 2^29 from the box origin,
-// so every coordinate within four box lengths fits and an image shift is 2^29 units.
-// Differences are taken modulo 2^32, exact for any pair closer than 4 L.
-constexpr double kFixUnits = 536870912.0;   // 2^29
+// 32-bit fixed point of a box coordinate: units of L / 2^30 from the box origin, so every
+// coordinate from one box length below the box to one above fits and an image shift is 2^30
+// units.  Differences are taken modulo 2^32, exact for any pair closer than 2 L.
+constexpr double kFixUnits = 1073741824.0;   // 2^30
 __device__ __forceinline__ int fix_coord(double v, double o, double units_per_length) {
     return __double2int_rn((v - o) * units_per_length);
 }
 __device__ __forceinline__ int fix_diff(int a, int b, int images) {
-    return (int)((unsigned)a - (unsigned)b + (unsigned)images * 536870912u);
+    return (int)((unsigned)a - (unsigned)b + (unsigned)images * 1073741824u);
 }
 
 // A density window reads slots [base, base + wn4) with base = min(a, kTileP - wn4), inside the
@@ -1032,9 +1032,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const float qy = (float)(T.L[1] / kFixUnits), qz = (float)(T.L[2] / kFixUnits);
     const int Yi = fix_coord(yi, T.o[1], ups_y), Zi = fix_coord(zi, T.o[2], ups_z);
     const float hif = (float)hi;
-    // a neighbour within reach more than ~4 box lengths away across a periodic image would
+    // a neighbour within reach 2 or more box lengths away across periodic images would
     // overflow the fixed-point difference: such waves take dy, dz from FP64 positions instead
-    const bool far_images = kFix && __ballot(rch >= 3.5 * fmin(T.L[1], T.L[2])) != 0ull;
+    const bool far_images = kFix && __ballot(rch >= 1.75 * fmin(T.L[1], T.L[2])) != 0ull;
 
     // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
     // range [ub, ue) of all lanes, this lane's own range [jl, jh) (empty if it does not need

@@ -169,7 +169,7 @@ def test_wide_averaging_cells_parity(oracle_mod, precision, avg_factor):
 def test_neighbours_many_periodic_images_away(oracle_mod):
     """A tube 0.001 wide with h = 0.006 in the shared state: neighbours sit up to 12 periodic
     images away in y and z, beyond the range of the FP32 force pass's fixed-point y, z
-    differences (4 box lengths), so those waves take dy, dz from FP64 positions.  One step in
+    differences (2 box lengths), so those waves take dy, dz from FP64 positions.  One step in
     mixed precision: sets bit-exact, fields within the mixed tolerances."""
     width, h = 0.001, 0.006
     spec = configs.tube(width, 0.01, 10.0, 1.0, max_steps=1, precision=MIXED)
