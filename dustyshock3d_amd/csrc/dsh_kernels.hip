@@ -598,21 +598,14 @@ __device__ __forceinline__ bool radius_window(const TubeConst& T, const TubeDyn&
 // within W_a of c along axis a (clamped on a bounded axis, wrapped on a periodic one; a
 // periodic axis shorter than the window is taken whole).  Max is exact, so three passes
 // give the same value as the full window.
-__global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __restrict__ tc,
-                                                    const TubeDyn* __restrict__ td, const double* __restrict__ in,
-                                                    double* __restrict__ out, int axis) {
-    const ItemPos ip = locate(mp, kBS);
-    const int li = ip.li + threadIdx.x;
-    if (li >= ip.sg.n) return;
-    const TubeConst& T = tc[ip.sg.tube];
-    if (T.h_mode == 0) return;
-    int w[3];
-    if (!radius_window(T, td[ip.sg.tube], ip.sg.species, w)) return;
+// Max of `in` over the window of w[axis] cells either side of local cell li along one axis.
+__device__ __forceinline__ double axis_window_max(const TubeConst& T, const int w[3], const double* __restrict__ in,
+                                                  int off, int li, int axis) {
     const int n0 = T.n[0], n1 = T.n[1];
     const int k[3] = {li % n0, (li / n0) % n1, li / (n0 * n1)};
     const int stride = axis == 0 ? 1 : (axis == 1 ? n0 : n0 * n1);
     const int n = T.n[axis], ka = k[axis];
-    const int base = ip.sg.off + li - ka * stride;
+    const int base = off + li - ka * stride;
     double m = 0.0;
     if (T.per[axis] && 2 * w[axis] + 1 >= n) {
         for (int c = 0; c < n; ++c) m = fmax(m, in[base + c * stride]);
@@ -626,16 +619,30 @@ __global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __r
         const int c0 = max(ka - w[axis], 0), c1 = min(ka + w[axis], n - 1);
         for (int c = c0; c <= c1; ++c) m = fmax(m, in[base + c * stride]);
     }
-    out[ip.sg.off + li] = m;
+    return m;
+}
+
+// The x and y passes of the separable window max (the z pass is taken inside k_radius).
+__global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __restrict__ tc,
+                                                    const TubeDyn* __restrict__ td, const double* __restrict__ in,
+                                                    double* __restrict__ out, int axis) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    if (li >= ip.sg.n) return;
+    const TubeConst& T = tc[ip.sg.tube];
+    if (T.h_mode == 0) return;
+    int w[3];
+    if (!radius_window(T, td[ip.sg.tube], ip.sg.species, w)) return;
+    out[ip.sg.off + li] = axis_window_max(T, w, in, ip.sg.off, li, axis);
 }
 
 // Reach of a cell: every neighbour j of a particle i of the cell has |x_i - x_j| < h_i + h_j
 // <= h_cell + h_j.  Cells more than W_a = ceil(2 h_max (1+m) / c_a) cells away along an
 // axis are at least W_a c_a >= 2 h_max apart, so h_j is bounded by the max of cell h over
-// that window (wmax, from the three k_window_max passes), and the pair kernels search
-// ceil(reach / c_a) cells along each axis.
+// that window (the x and y passes of k_window_max, then the z pass here), and the pair
+// kernels search ceil(reach / c_a) cells along each axis.
 __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                const double* __restrict__ cellh, const double* __restrict__ wmax,
+                                                const double* __restrict__ cellh, const double* __restrict__ wxy,
                                                 double* __restrict__ reach) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
@@ -659,7 +666,7 @@ __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restr
         reach[c] = 2.0 * hs * (1.0 + kRadiusMargin);
         return;
     }
-    reach[c] = (hc + wmax[c]) * (1.0 + kRadiusMargin);
+    reach[c] = (hc + axis_window_max(T, w, wxy, ip.sg.off, li, 2)) * (1.0 + kRadiusMargin);
 }
 
 template <typename R, bool kStats>
@@ -2071,8 +2078,7 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     const dim3 g(a.cmap.nblocks), b(kBS);
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, 0);
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp, a.wtmp + a.ncells, 1);
-    hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp + a.ncells, a.wtmp, 2);
-    hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, a.reach);
+    hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp + a.ncells, a.reach);
     return hipGetLastError();
 }
 
