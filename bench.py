@@ -61,7 +61,7 @@ def pass_kernels(stage, tile=DEFAULT_TILE):
     return {
         "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
         "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
-        "idic": ["k_idic_runs"],
+        "idic": ["k_idic_runs<1>"],
         "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
                  "k_window_max", "k_radius"],
     }.get(stage, [])

@@ -1817,8 +1817,10 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
 // j sums cell j's in sorted order, solves cell j and leaves its drag terms in LDS for the
 // particles of the cell.  One conservation / CFL partial per block (its runs in order).
 // The kernel is latency-bound (a chain of dependent loads, sums and FP64 divisions per wave),
-// so registers are capped for 8 waves per SIMD: more waves in flight beat the few spills.
+// so registers are capped for 8 waves per SIMD.  The number of dust species is a template
+// parameter (one instance per count present), so registers hold only the sums in use.
 constexpr int kRunCells = 8;
+template <int kNd>
 __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
                                                    const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                                    const int* __restrict__ start, const int* __restrict__ ginv,
@@ -1833,20 +1835,19 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
     const TubeDyn& D = td[t];
     if (D.done || D.err) return;   // the block's cells are all of tube t
     const TubeConst& T = tc[t];
-    if (T.f != 1) return;   // k_idic
+    if (T.f != 1 || T.nsp != kNd + 1) return;   // k_idic, or another instance
     double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
     if (la0 < ip.sg.n) {
         const long long wo = (long long)(1 - *cur) * B.ntot;
         const int ncl = min(kRunCells, ip.sg.n - la0);
-        const int nd = T.nsp - 1;
+        constexpr int nd = kNd;
         const double tau = D.dt;
         double (*V)[64] = sv[wave];
 
         // lane j < ncl: sums of cell la0 + j
         CellSums S{};
 #pragma unroll
-        for (int s = 0; s < kMaxSpecies; ++s) {
-            if (s >= T.nsp) break;
+        for (int s = 0; s <= kNd; ++s) {
             const int c0 = T.cell_off[s] + la0;
             const int p0 = start[c0], p1 = start[c0 + ncl];
             const int jlo = lane < ncl ? start[c0 + lane] : 0, jhi = lane < ncl ? start[c0 + lane + 1] : 0;
@@ -1887,8 +1888,7 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
         __builtin_amdgcn_wave_barrier();
 
 #pragma unroll
-        for (int s = 0; s < kMaxSpecies; ++s) {
-            if (s >= T.nsp) break;
+        for (int s = 0; s <= kNd; ++s) {
             const int c0 = T.cell_off[s] + la0;
             const int p0 = start[c0], p1 = start[c0 + ncl];
             for (int k = p0 + lane; k < p1; k += 64) {
@@ -2152,8 +2152,14 @@ hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
     if (a.idic_blocks)
         hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
                            a.ginv, a.part);
-    if (a.idic_runs)
-        hipLaunchKernelGGL(k_idic_runs, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
+    if (a.idic_runs & 1u)
+        hipLaunchKernelGGL(k_idic_runs<0>, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
+                           a.B, a.start, a.ginv, a.part);
+    if (a.idic_runs & 2u)
+        hipLaunchKernelGGL(k_idic_runs<1>, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
+                           a.B, a.start, a.ginv, a.part);
+    if (a.idic_runs & 4u)
+        hipLaunchKernelGGL(k_idic_runs<2>, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
                            a.B, a.start, a.ginv, a.part);
     return hipGetLastError();
 }
