@@ -258,16 +258,12 @@ int Engine::prepare() {
             coff += C.ncell;
             if (poff > 2000000000ll || coff > 2000000000ll) return fail(DSH_ERR_ARG, "batch too large");
         }
-    a_.idic_blocks = false;
-    a_.idic_runs = 0u;
+    a_.idic_blocks = a_.idic_runs = 0u;
     for (int t = 0; t < T; ++t) {
         TubeConst& C = tc_h_[t];
         C.avg_off = (int)aoff;
         C.npart = C.f == 1 ? (C.navg + idic_block_cells() - 1) / idic_block_cells() : C.navg;
-        if (C.f == 1)
-            a_.idic_runs |= 1u << (C.nsp - 1);
-        else
-            a_.idic_blocks = true;
+        (C.f == 1 ? a_.idic_runs : a_.idic_blocks) |= 1u << (C.nsp - 1);
         aseg_.push_back(Seg{(int)aoff, C.navg, t, 0});
         aoff += C.navg;
     }

@@ -54,7 +54,7 @@ struct StepArgs {
     Map pmap, gmap, cmap, amap;
     Map pmap64, gmap64;    // 64 targets per block (tiled pair kernels)
     Map amap_runs;         // idic_block_cells() averaging cells per block, idic_run_cells() per wave (f == 1)
-    bool idic_blocks;              // a tube with averaging cells of f^3 > 1 neighbour cells
+    unsigned idic_blocks;          // bit nd: a tube with averaging cells of f^3 > 1 neighbour cells, nd dust species
     unsigned idic_runs;            // bit nd: a tube with f == 1 and nd dust species
     unsigned long long* masks;   // gas density hit masks reused by the force pass
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null

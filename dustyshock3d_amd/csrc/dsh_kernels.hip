@@ -1739,6 +1739,7 @@ __device__ __forceinline__ double dust_tstop(const TubeConst& T, const PBuf& B, 
 // Averaging cells of f^3 neighbour cells (f >= 2): one wave per averaging cell, sums over
 // its neighbour cells for every species, implicit drag solve, then the per-particle kick,
 // drift, energy and h update, and the cell's conservation / CFL partials.
+template <int kNd>
 __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restrict__ tc,
                                               const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                               const int* __restrict__ start, const int* __restrict__ ginv,
@@ -1751,19 +1752,18 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     const TubeDyn& D = td[t];
     if (D.done || D.err) return;
     const TubeConst& T = tc[t];
-    if (T.f == 1) return;   // k_idic_runs
+    if (T.f == 1 || T.nsp != kNd + 1) return;   // k_idic_runs, or another instance
     const long long wo = (long long)(1 - *cur) * B.ntot;
     const int ax_ = la % T.na[0];
     const int ay_ = (la / T.na[0]) % T.na[1];
     const int az_ = la / (T.na[0] * T.na[1]);
-    const int nd = T.nsp - 1;
+    constexpr int nd = kNd;
     const double tau = D.dt;
 
     CellSums S{};
     // species loops unrolled: the per-species arrays are then indexed by constants
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) {
-        if (s >= T.nsp) break;
+    for (int s = 0; s <= kNd; ++s) {
         for_avg_cell(T, start, s, ax_, ay_, az_, lane, [&](int k) {
             if (s == 0) {
                 ++S.ng;
@@ -1791,8 +1791,7 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
     const CellDrag C = cell_drag(T, nd, tau, S);
     double px = 0, py = 0, pz = 0, E = 0, mn = 1e300;
 #pragma unroll
-    for (int s = 0; s < kMaxSpecies; ++s) {
-        if (s >= T.nsp) break;
+    for (int s = 0; s <= kNd; ++s) {
         for_avg_cell(T, start, s, ax_, ay_, az_, lane,
                      [&](int k) { particle_update(T, B, wo + k, s, nd, tau, C, px, py, pz, E, mn); });
     }
@@ -2149,8 +2148,14 @@ hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
 }
 
 hipError_t launch_idic(const StepArgs& a, hipStream_t s) {
-    if (a.idic_blocks)
-        hipLaunchKernelGGL(k_idic, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
+    if (a.idic_blocks & 1u)
+        hipLaunchKernelGGL(k_idic<0>, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
+                           a.ginv, a.part);
+    if (a.idic_blocks & 2u)
+        hipLaunchKernelGGL(k_idic<1>, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
+                           a.ginv, a.part);
+    if (a.idic_blocks & 4u)
+        hipLaunchKernelGGL(k_idic<2>, dim3(a.amap.nblocks), dim3(kBS), 0, s, a.amap, a.tc, a.td, a.cur, a.B, a.start,
                            a.ginv, a.part);
     if (a.idic_runs & 1u)
         hipLaunchKernelGGL(k_idic_runs<0>, dim3(a.amap_runs.nblocks), dim3(kBS), 0, s, a.amap_runs, a.tc, a.td, a.cur,
