@@ -1039,9 +1039,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const float qy = (float)(T.L[1] / kFixUnits), qz = (float)(T.L[2] / kFixUnits);
     const int Yi = fix_coord(yi, T.o[1], ups_y), Zi = fix_coord(zi, T.o[2], ups_z);
     const float hif = (float)hi;
-    // a neighbour within reach 2 or more box lengths away across periodic images would
-    // overflow the fixed-point difference: such waves take dy, dz from FP64 positions instead
-    const bool far_images = kFix && __ballot(rch >= 1.75 * fmin(T.L[1], T.L[2])) != 0ull;
+    // a neighbour within reach 2 or more box lengths away across periodic images, or a
+    // coordinate outside the fixed-point range (one box length either side of the box), would
+    // break the fixed-point difference: such waves (or pieces) take dy, dz from FP64 positions
+    auto fix_range_ok = [&](double y, double z) {
+        const double uy = (y - T.o[1]) * ups_y, uz = (z - T.o[2]) * ups_z;
+        return uy > -0.95 * kFixUnits && uy < 1.95 * kFixUnits && uz > -0.95 * kFixUnits && uz < 1.95 * kFixUnits;
+    };
+    const bool far_images =
+        kFix && __ballot(rch >= 1.75 * fmin(T.L[1], T.L[2]) || !fix_range_ok(yi, zi)) != 0ull;
+    bool far_piece = far_images;
 
     // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
     // range [ub, ue) of all lanes, this lane's own range [jl, jh) (empty if it does not need
@@ -1056,6 +1063,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     // keeps the compiler from serialising them under register pressure).
     auto stage = [&](int s0, int n0, int s1, int n1) {
         __syncthreads();
+        bool out_of_range = false;
         for (int k = lane; k < n0 + n1; k += 64) {
             const int src = k < n0 ? s0 + k : s1 + (k - n0);
             const long long g = wo + src;
@@ -1078,6 +1086,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 L.y[k] = fix_coord(y, T.o[1], ups_y);
                 L.z[k] = fix_coord(z, T.o[2], ups_z);
                 L.h[k] = (float)h;
+                out_of_range = out_of_range || !fix_range_ok(y, z);
             } else {
                 L.y[k] = y;
                 L.z[k] = z;
@@ -1098,6 +1107,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             }
             if (kStats) L.id[k] = id;
         }
+        if constexpr (kFix) far_piece = far_images || __ballot(out_of_range) != 0ull;
         __syncthreads();
     };
     // Hit mask of one window of 64 candidates of this lane's range, starting at staged slot
@@ -1202,7 +1212,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 // image is sy, sz box lengths); every term is antisymmetric in (i, j)
                 const float fx = (float)dx;
                 float fy, fz, dd;
-                if (!far_images) {
+                if (!far_piece) {
                     fy = (float)fix_diff(L.y[u], Yi, sy) * qy;
                     fz = (float)fix_diff(L.z[u], Zi, sz) * qz;
                     dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
