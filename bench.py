@@ -172,7 +172,7 @@ def main():
     value = (n_gas + n_dust) * steps_done / wall
 
     groups = {"sort": sum(kt.get(k, 0.0) for k in SORT_STAGES)}
-    for k in ("density", "force", "idic", "finalize", "flip"):
+    for k in ("density", "force", "idic", "finalize"):
         groups[k] = kt.get(k, 0.0)
     dominant = max(("density", "force", "idic", "sort"), key=lambda k: groups[k])
     dom_ms = groups[dominant]

@@ -345,14 +345,14 @@ int Engine::prepare() {
     TubeDyn* dtd;
     if ((rc = alloc(&dtc, (size_t)T))) return rc;
     if ((rc = alloc(&dtd, (size_t)T))) return rc;
-    if ((rc = alloc(&a_.cur, 1))) return rc;
+    if ((rc = alloc(&a_.cur, 2))) return rc;   // [0] current buffer, [1] finalize ticket
     a_.tc = dtc;
     a_.td = dtd;
     a_.ntubes = T;
     if ((rc = hip_check(hipMemcpy(dtc, tc_h_.data(), sizeof(TubeConst) * T, hipMemcpyHostToDevice), "H2D tubes")))
         return rc;
-    const int zero = 0;
-    if ((rc = hip_check(hipMemcpy(a_.cur, &zero, sizeof(int), hipMemcpyHostToDevice), "H2D cur"))) return rc;
+    const int zero[2] = {0, 0};
+    if ((rc = hip_check(hipMemcpy(a_.cur, zero, sizeof(zero), hipMemcpyHostToDevice), "H2D cur"))) return rc;
 
     // adopted tubes: device-to-device copy of the source's current buffer (its sorted
     // order and ids are kept; the first step re-sorts to the same order)
@@ -451,7 +451,7 @@ int Engine::prepare() {
 namespace {
 // rank_permute also takes each cell's largest h
 const char* const kStageNames[] = {"begin",   "key",   "scan", "scatter",  "rank_permute", "radius",
-                                   "density", "force", "idic", "finalize", "flip"};
+                                   "density", "force", "idic", "finalize"};
 constexpr int kStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 }  // namespace
 
@@ -481,7 +481,6 @@ int Engine::enqueue_step(bool timed) {
             case 7: e = launch_force(a_, precision_, stream_); break;
             case 8: e = launch_idic(a_, stream_); break;
             case 9: e = launch_finalize(a_, stream_); break;
-            case 10: e = launch_flip(a_, stream_); break;
         }
         if (e != hipSuccess) return hip_check(e, kStageNames[i]);
         if (timed) (void)hipEventRecord(ev[i + 1], stream_);

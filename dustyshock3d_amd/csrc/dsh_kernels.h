@@ -31,7 +31,7 @@ inline bool valid_tile(int t) {
 struct StepArgs {
     const TubeConst* tc;
     TubeDyn* td;
-    int* cur;              // device: buffer index of the current state
+    int* cur;              // device: [0] buffer index of the current state, [1] finalize's block ticket
     PBuf B;
     int ntubes;
     // per particle (global sorted space)
@@ -76,7 +76,6 @@ hipError_t launch_idic(const StepArgs& a, hipStream_t s);
 int idic_run_cells();     // averaging cells per wave of the f == 1 drag kernel
 int idic_block_cells();   // averaging cells per block of it (one conservation partial each)
 hipError_t launch_finalize(const StepArgs& a, hipStream_t s);
-hipError_t launch_flip(const StepArgs& a, hipStream_t s);
 // seeded shock-tube set-up of one (tube, species) straight into buffer slots off..off+n-1
 hipError_t launch_tube_ic(const TubeIC& c, const PBuf& B, long long off, hipStream_t s);
 

@@ -1933,38 +1933,12 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
 }
 
-// One block per tube: deterministic reduction of the averaging-cell partials, then time,
-// next step size and completion (the loop control of kernel.cu:1929-1932,2566 plus CFL).
 constexpr int kFinBS = 1024;   // threads of the per-tube reduction
-__global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                     const int* __restrict__ cur, const double* __restrict__ part) {
-    __shared__ double sh[5][kFinBS / 64];
-    __shared__ int skip;
-    const int t = blockIdx.x;
-    const TubeConst& T = tc[t];
-    TubeDyn& D = td[t];
-    if (threadIdx.x == 0) skip = D.done;
-    __syncthreads();
-    if (skip) return;
-    const bool failed = D.err != 0;
-    double a[4] = {0, 0, 0, 0}, mn = 1e300;
-    if (!failed) {
-        for (int i = threadIdx.x; i < T.npart; i += kFinBS) {
-            const double* o = part + 5ll * (T.avg_off + i);
-            for (int c = 0; c < 4; ++c) a[c] += o[c];
-            mn = fmin(mn, o[4]);
-        }
-    }
-    for (int c = 0; c < 4; ++c) a[c] = wave_sum(a[c]);
-    mn = wave_min(mn);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        for (int c = 0; c < 4; ++c) sh[c][w] = a[c];
-        sh[4][w] = mn;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    const int next_cur = 1 - *cur;
+
+// Time, next step size and completion of one tube from its reduced sums (the loop control of
+// kernel.cu:1929-1932,2566 plus CFL); sh holds the per-wave sums and minima.
+__device__ void finish_tube(const TubeConst& T, TubeDyn& D, int next_cur, bool failed,
+                            const double (*sh)[kFinBS / 64]) {
     if (failed) {
         D.done = 1;
         D.done_cur = next_cur;
@@ -2012,7 +1986,46 @@ __global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict
     }
 }
 
-__global__ void k_flip(int* cur) { *cur = 1 - *cur; }
+// One block per tube: deterministic reduction of the averaging-cell partials, then
+// finish_tube.  The last block to finish flips the current-buffer flag cur[0] for the next
+// step (cur[1] counts finished blocks and is reset by that block).
+__global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
+                                                     int* __restrict__ cur, const double* __restrict__ part) {
+    __shared__ double sh[5][kFinBS / 64];
+    __shared__ int skip;
+    const int t = blockIdx.x;
+    const TubeConst& T = tc[t];
+    TubeDyn& D = td[t];
+    if (threadIdx.x == 0) skip = D.done;
+    __syncthreads();
+    if (!skip) {
+        const bool failed = D.err != 0;
+        double a[4] = {0, 0, 0, 0}, mn = 1e300;
+        if (!failed) {
+            for (int i = threadIdx.x; i < T.npart; i += kFinBS) {
+                const double* o = part + 5ll * (T.avg_off + i);
+                for (int c = 0; c < 4; ++c) a[c] += o[c];
+                mn = fmin(mn, o[4]);
+            }
+        }
+        for (int c = 0; c < 4; ++c) a[c] = wave_sum(a[c]);
+        mn = wave_min(mn);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            for (int c = 0; c < 4; ++c) sh[c][w] = a[c];
+            sh[4][w] = mn;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) finish_tube(T, D, 1 - cur[0], failed, sh);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();   // this block's reads of cur[0] come before its ticket
+        if (atomicAdd(&cur[1], 1) == (int)gridDim.x - 1) {
+            cur[1] = 0;
+            cur[0] = 1 - cur[0];
+        }
+    }
+}
 
 // The seeded lattice set-up of one (tube, species), one thread per particle, written in
 // original order with id = index (what an upload of dsh_tube_ic's arrays would store).
@@ -2190,11 +2203,6 @@ hipError_t launch_finalize(const StepArgs& a, hipStream_t s) {
 hipError_t launch_tube_ic(const TubeIC& c, const PBuf& B, long long off, hipStream_t s) {
     if (c.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_tube_ic, grid1(c.n), dim3(kBS), 0, s, c, B, off);
-    return hipGetLastError();
-}
-
-hipError_t launch_flip(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_flip, dim3(1), dim3(1), 0, s, a.cur);
     return hipGetLastError();
 }
 
