@@ -39,7 +39,7 @@ ALGO_BYTES = {
 }
 AVG_CELL_BYTES = 9
 DEFAULT_TILE = 192   # the library's default piece size of the tiled pair kernels
-SORT_STAGES = ("begin", "key", "scan", "scatter", "rank_permute", "radius")
+SORT_STAGES = ("key", "scan", "scatter", "rank_permute", "radius")
 
 
 def stage_bytes(stage, n_gas, n_dust, n_avg):
@@ -62,7 +62,7 @@ def pass_kernels(stage, tile=DEFAULT_TILE):
         "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
         "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
         "idic": ["k_idic_runs<1>"],
-        "sort": ["k_begin", "k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
+        "sort": ["k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
                  "k_window_max", "k_radius"],
     }.get(stage, [])
 

@@ -336,6 +336,10 @@ int Engine::prepare() {
     if ((rc = alloc(&a_.count, (size_t)coff))) return rc;
     if ((rc = alloc(&a_.start, (size_t)coff + 1))) return rc;
     if ((rc = alloc(&a_.cellh, (size_t)coff))) return rc;
+    // cell counts and cell maxima of h start at zero; each step's scan and radius passes clear
+    // them again after use
+    if ((rc = hip_check(hipMemset(a_.count, 0, (size_t)coff * sizeof(int)), "clear counts"))) return rc;
+    if ((rc = hip_check(hipMemset(a_.cellh, 0, (size_t)coff * sizeof(double)), "clear cell h"))) return rc;
     if ((rc = alloc(&a_.reach, (size_t)coff))) return rc;
     if ((rc = alloc(&a_.wtmp, 2 * (size_t)coff))) return rc;
     a_.scan_nb = scan_blocks((int)coff);
@@ -450,8 +454,8 @@ int Engine::prepare() {
 
 namespace {
 // rank_permute also takes each cell's largest h
-const char* const kStageNames[] = {"begin",   "key",   "scan", "scatter",  "rank_permute", "radius",
-                                   "density", "force", "idic", "finalize"};
+const char* const kStageNames[] = {"key",     "scan",  "scatter", "rank_permute", "radius",
+                                   "density", "force", "idic",    "finalize"};
 constexpr int kStages = (int)(sizeof(kStageNames) / sizeof(kStageNames[0]));
 }  // namespace
 
@@ -471,16 +475,15 @@ int Engine::enqueue_step(bool timed) {
     for (int i = 0; i < kStages; ++i) {
         hipError_t e = hipSuccess;
         switch (i) {
-            case 0: e = launch_begin(a_, stream_); break;
-            case 1: e = launch_key(a_, stream_); break;
-            case 2: e = launch_scan(a_, stream_); break;
-            case 3: e = launch_scatter(a_, stream_); break;
-            case 4: e = launch_rank_permute(a_, stream_); break;
-            case 5: e = launch_radius(a_, stream_); break;
-            case 6: e = launch_density(a_, precision_, stats_, stream_); break;
-            case 7: e = launch_force(a_, precision_, stream_); break;
-            case 8: e = launch_idic(a_, stream_); break;
-            case 9: e = launch_finalize(a_, stream_); break;
+            case 0: e = launch_key(a_, stream_); break;
+            case 1: e = launch_scan(a_, stream_); break;
+            case 2: e = launch_scatter(a_, stream_); break;
+            case 3: e = launch_rank_permute(a_, stream_); break;
+            case 4: e = launch_radius(a_, stream_); break;
+            case 5: e = launch_density(a_, precision_, stats_, stream_); break;
+            case 6: e = launch_force(a_, precision_, stream_); break;
+            case 7: e = launch_idic(a_, stream_); break;
+            case 8: e = launch_finalize(a_, stream_); break;
         }
         if (e != hipSuccess) return hip_check(e, kStageNames[i]);
         if (timed) (void)hipEventRecord(ev[i + 1], stream_);

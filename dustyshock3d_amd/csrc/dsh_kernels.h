@@ -64,7 +64,6 @@ struct StepArgs {
 
 // scratch: 5 doubles per particle block of pmap
 hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s);
-hipError_t launch_begin(const StepArgs& a, hipStream_t s);
 hipError_t launch_key(const StepArgs& a, hipStream_t s);
 hipError_t launch_scan(const StepArgs& a, hipStream_t s);
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s);

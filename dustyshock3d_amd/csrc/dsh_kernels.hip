@@ -319,18 +319,6 @@ __global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __rest
     }
 }
 
-__global__ __launch_bounds__(kBS) void k_begin(int* __restrict__ count, double* __restrict__ cellh, int ncells,
-                                               TubeDyn* __restrict__ td, int ntubes) {
-    const int i = blockIdx.x * kBS + threadIdx.x;
-    if (i < ncells) {
-        count[i] = 0;
-        cellh[i] = 0.0;
-    }
-    if (i < ntubes) {
-        for (int s = 0; s < kMaxSpecies; ++s) td[i].hmax_bits[s] = 0ull;
-    }
-}
-
 // Cell key and slot.  Consecutive lanes hold consecutive particles of the previous
 // step's sorted order, so equal keys come in runs: one atomic per run, found with a
 // wave ballot of run heads.
@@ -415,7 +403,9 @@ __global__ __launch_bounds__(kBS) void k_scan_top(int* __restrict__ bsum, int nb
     }
 }
 
-__global__ __launch_bounds__(kBS) void k_scan_down(const int* __restrict__ in, int n, const int* __restrict__ bsum,
+// The counts are consumed here: each is cleared after it is read, ready for the next step's
+// k_key.
+__global__ __launch_bounds__(kBS) void k_scan_down(int* __restrict__ in, int n, const int* __restrict__ bsum,
                                                    int* __restrict__ out) {
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -423,6 +413,7 @@ __global__ __launch_bounds__(kBS) void k_scan_down(const int* __restrict__ in, i
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         v[k] = (base + k < n) ? in[base + k] : 0;
+        if (base + k < n) in[base + k] = 0;
         s += v[k];
     }
     int tot;
@@ -465,7 +456,7 @@ __global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__
 // block (called by every thread): keys ascend along a wave, so a segmented max over runs of
 // equal keys leaves each run's maximum on its last lane, which raises the cell's value
 // (h > 0, so the FP64 bits order like the values).  Cells without particles keep the 0
-// written by k_begin.
+// they start the step with (cleared at set-up and by k_radius).
 __device__ __forceinline__ void cell_max_h(const ItemPos& ip, bool valid, int k, double h,
                                            double* __restrict__ cellh, TubeDyn* __restrict__ td) {
     __shared__ double wmax[kBS / 64];
@@ -641,8 +632,10 @@ __global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __r
 // axis are at least W_a c_a >= 2 h_max apart, so h_j is bounded by the max of cell h over
 // that window (the x and y passes of k_window_max, then the z pass here), and the pair
 // kernels search ceil(reach / c_a) cells along each axis.
+// The cell maxima of h are consumed here (the window passes have read them): each is cleared
+// after it is read, ready for the next step's rank pass.
 __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
-                                                const double* __restrict__ cellh, const double* __restrict__ wxy,
+                                                double* __restrict__ cellh, const double* __restrict__ wxy,
                                                 double* __restrict__ reach) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
@@ -651,6 +644,7 @@ __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restr
     const TubeConst& T = tc[ip.sg.tube];
     TubeDyn& D = td[ip.sg.tube];
     const double hc = cellh[c];
+    cellh[c] = 0.0;
     if (hc <= 0.0) {
         reach[c] = 0.0;
         return;
@@ -2019,6 +2013,8 @@ __global__ __launch_bounds__(kFinBS) void k_finalize(const TubeConst* __restrict
         if (threadIdx.x == 0) finish_tube(T, D, 1 - cur[0], failed, sh);
     }
     if (threadIdx.x == 0) {
+        // the per-species max of h is rebuilt by the next step's rank pass
+        for (int s = 0; s < kMaxSpecies; ++s) D.hmax_bits[s] = 0ull;
         __threadfence();   // this block's reads of cur[0] come before its ticket
         if (atomicAdd(&cur[1], 1) == (int)gridDim.x - 1) {
             cur[1] = 0;
@@ -2062,12 +2058,6 @@ size_t mask_words(int gas_chunks) { return (size_t)gas_chunks * kMaskSlots * 64 
 hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s) {
     hipLaunchKernelGGL(k_prepare_part, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.B, scratch);
     hipLaunchKernelGGL(k_prepare, dim3(a.ntubes), dim3(kBS), 0, s, a.pmap, a.tc, a.td, scratch);
-    return hipGetLastError();
-}
-
-hipError_t launch_begin(const StepArgs& a, hipStream_t s) {
-    const int n = a.ncells > a.ntubes ? a.ncells : a.ntubes;
-    hipLaunchKernelGGL(k_begin, grid1(n), dim3(kBS), 0, s, a.count, a.cellh, a.ncells, a.td, a.ntubes);
     return hipGetLastError();
 }
 
