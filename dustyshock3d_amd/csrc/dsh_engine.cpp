@@ -325,10 +325,9 @@ int Engine::prepare() {
     }
     if ((rc = alloc(&B.ind, n2))) return rc;
     if ((rc = alloc(&B.id, n2))) return rc;
-    int** ifields[] = {&a_.key, &a_.slot, &a_.tsrc, &a_.tid, &a_.tkey, &a_.wkey, &a_.ncount};
+    int** ifields[] = {&a_.key, &a_.slot, &a_.tsrc, &a_.tkey, &a_.wkey, &a_.ncount};
     for (int** f : ifields)
         if ((rc = alloc(f, (size_t)ntot_))) return rc;
-    if ((rc = alloc(&a_.tx, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.pr2, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.csav, (size_t)ntot_))) return rc;
     if ((rc = alloc(&a_.nhash, (size_t)ntot_))) return rc;

@@ -35,8 +35,7 @@ struct StepArgs {
     PBuf B;
     int ntubes;
     // per particle (global sorted space)
-    int *key, *slot, *tsrc, *tid, *tkey, *wkey;
-    unsigned long long* tx;  // x in scatter order as ordered bits (the sort's secondary key)
+    int *key, *slot, *tsrc, *tkey, *wkey;
     double *pr2, *csav;
     int* ncount;
     unsigned long long* nhash;

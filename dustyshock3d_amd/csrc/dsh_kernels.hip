@@ -1,20 +1,21 @@
 // dsh_kernels.hip - gfx950 kernels of one dusty-gas SPH step over a batch of tubes.
 //
 // Step (one stream, no host round trips):
-//   begin        zero cell counts and per-tube maxima
 //   key          neighbour-cell key per particle; wave-aggregated atomic slot in its cell
-//   scan         exclusive scan of cell counts -> cell starts (global over all segments)
-//   scatter      particle -> (cell start + slot)
-//   rank_permute rank inside the cell by (x, id), gather the state into the work buffer,
-//                max h per cell and per (species, tube); the sorted order is (cell key, x,
-//                id), and the reference's linked lists (cell, id) are rebuilt at download
-//   radius       per-cell search radius in cells, from a window max of cell h
+//   scan         exclusive scan of cell counts -> cell starts (global over all segments);
+//                clears the counts for the next step
+//   scatter      particle index and key -> (cell start + slot)
+//   rank_permute rank inside the cell by (x, id) (gathered from the state), gather the state
+//                into the work buffer, max h per cell and per (species, tube); the sorted
+//                order is (cell key, x, id), and the reference's linked lists (cell, id) are
+//                rebuilt at download
+//   radius       per-cell search radius in cells, from a window max of cell h (clears it)
 //   density      same-species density sums; pressure, sound speed, stopping time
 //   force        gas pressure + artificial-viscosity acceleration and de/dt
 //   idic         per averaging cell: sums, implicit two-fluid drag solve (FP64), per-particle
 //                velocity update, drift, energy, smoothing length, conservation partials
-//   finalize     per tube: conservation sums, time, next CFL step, completion
-//   flip         swap state and work buffers
+//   finalize     per tube: conservation sums, time, next CFL step, completion; its last
+//                block swaps the state and work buffers
 #include "dsh_kernels.h"
 
 #include <hip/hip_runtime.h>
@@ -433,22 +434,17 @@ __device__ __forceinline__ unsigned long long order_bits(double v) {
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-__global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__ cur, PBuf B,
-                                                 const int* __restrict__ key, const int* __restrict__ slot,
+__global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__ key, const int* __restrict__ slot,
                                                  const int* __restrict__ start, int* __restrict__ tsrc,
-                                                 int* __restrict__ tid, int* __restrict__ tkey,
-                                                 unsigned long long* __restrict__ tx) {
+                                                 int* __restrict__ tkey) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
     const int p = ip.sg.off + li;
-    const long long so = (long long)(*cur) * B.ntot;
     const int k = key[p];
     const int pos = start[k] + slot[p];
     tsrc[pos] = p;
-    tid[pos] = B.id[so + p];
     tkey[pos] = k;
-    tx[pos] = order_bits(B.x[so + p]);
 }
 
 
@@ -491,9 +487,7 @@ constexpr int kRankRun = 1024;   // cell members staged per block by k_rank_perm
 __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restrict__ td,
                                                       const int* __restrict__ cur, PBuf B,
                                                       const int* __restrict__ start, const int* __restrict__ tsrc,
-                                                      const int* __restrict__ tid, const int* __restrict__ tkey,
-                                                      const unsigned long long* __restrict__ tx,
-                                                      int* __restrict__ wkey, int* __restrict__ ginv,
+                                                      const int* __restrict__ tkey, int* __restrict__ wkey, int* __restrict__ ginv,
                                                       double* __restrict__ cellh) {
     // the block's positions lie in one segment and keys ascend along them, so the members of
     // every cell the block touches form one run [w0, w1), staged in LDS when it fits
@@ -505,10 +499,12 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
     const int ql = ip.sg.off + min(ip.li + kBS, ip.sg.n) - 1;
     const int w0 = start[tkey[qf]], w1 = start[tkey[ql] + 1];
     const bool staged = w1 - w0 <= kRankRun;
+    const long long so = (long long)(*cur) * B.ntot;
     if (staged) {
         for (int k = threadIdx.x; k < w1 - w0; k += kBS) {
-            sx[k] = tx[w0 + k];
-            sid[k] = tid[w0 + k];
+            const long long g = so + tsrc[w0 + k];
+            sx[k] = order_bits(B.x[g]);
+            sid[k] = B.id[g];
         }
     }
     __syncthreads();
@@ -519,20 +515,20 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
     const int q = ip.sg.off + li;
     c = tkey[q];
     const int lo = start[c], hi = start[c + 1];
-    const int my = tid[q];
-    const unsigned long long mx = tx[q];
+    const int src = tsrc[q];
+    const int my = B.id[so + src];
+    const unsigned long long mx = order_bits(B.x[so + src]);
     int r = 0;
     if (staged) {
         for (int k = lo - w0; k < hi - w0; ++k) r += (sx[k] < mx || (sx[k] == mx && sid[k] < my)) ? 1 : 0;
     } else {
         for (int k = lo; k < hi; ++k) {
-            const unsigned long long kx = tx[k];
-            r += (kx < mx || (kx == mx && tid[k] < my)) ? 1 : 0;
+            const long long g = so + tsrc[k];
+            const unsigned long long kx = order_bits(B.x[g]);
+            r += (kx < mx || (kx == mx && B.id[g] < my)) ? 1 : 0;
         }
     }
     const int dst = lo + r;
-    const int src = tsrc[q];
-    const long long so = (long long)(*cur) * B.ntot;
     const long long wo = (long long)(1 - *cur) * B.ntot;
     B.x[wo + dst] = B.x[so + src];
     B.y[wo + dst] = B.y[so + src];
@@ -2075,14 +2071,14 @@ hipError_t launch_scan(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.cur, a.B, a.key, a.slot,
-                       a.start, a.tsrc, a.tid, a.tkey, a.tx);
+    hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.key, a.slot, a.start, a.tsrc,
+                       a.tkey);
     return hipGetLastError();
 }
 
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_rank_permute, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.cur, a.B, a.start,
-                       a.tsrc, a.tid, a.tkey, a.tx, a.wkey, a.ginv, a.cellh);
+                       a.tsrc, a.tkey, a.wkey, a.ginv, a.cellh);
     return hipGetLastError();
 }
 

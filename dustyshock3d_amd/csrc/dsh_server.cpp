@@ -47,7 +47,7 @@ double tube_bytes(const dsh_tube_spec& spec) {
     TubeConst C{};
     build_grid(p, hmin, C);
     const double particles = (double)n[0] + (double)n[1];
-    const double per_particle = 2.0 * 16 * 8 + 2.0 * 2 * 4 + 7 * 4 + 3 * 8 + 8;
+    const double per_particle = 2.0 * 16 * 8 + 2.0 * 2 * 4 + 6 * 4 + 3 * 8;
     const double masks = std::ceil(n[0] / 64.0) * 48.0 * 64.0 * 12.0;   // hit masks + x-windows
     const double cells = 2.0 * C.ncell * (4 + 4 + 8 + 8 + 16);
     const double avg = 40.0 * C.navg;
