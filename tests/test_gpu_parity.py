@@ -194,3 +194,35 @@ def test_neighbours_many_periodic_images_away(oracle_mod):
             e_a = vec_normwise(rg, ro, ("ax", "ay", "az"))
             print(f"gas A {e_a:.3e}")
             assert e_a <= 1e-5
+
+
+def test_dense_cells_parity(oracle_mod):
+    """The cfg0 tube with h = 0.025 in the shared state: cells of ~0.05 hold up to ~1060
+    members (the rank pass ranks from memory instead of LDS for runs over 1024) and each
+    particle has ~3600 neighbours (the force pass runs out of stored hit masks and falls back
+    to its own predicate).  One step in mixed precision: keys, order and sets bit-exact,
+    fields within the mixed tolerances."""
+    spec = configs.cfg0(max_steps=1)
+    o0 = oracle_mod.OracleSim.from_spec(spec)
+    params, state = o0.params, take_state(o0)
+    for s in (0, 1):
+        state[s]["h"] = np.full_like(state[s]["h"], 0.025)
+    o, g = _one_step_both(oracle_mod, params, state, MIXED)
+    for s in (0, 1):
+        ro, rg = o.download(s), g.download(s)
+        assert np.array_equal(rg["cell"], ro["cell"])
+        assert np.array_equal(rg["nn"], ro["nn"])
+        assert np.unique(ro["cell"], return_counts=True)[1].max() > 1024
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co) and np.array_equal(hg, ho)
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"\ndense species {s}: neighbours {float(cg.mean()):.0f} per particle, rho {e_rho:.3e} v {e_v:.3e}")
+        assert e_rho <= 1e-5
+        assert e_v <= 1e-5
+        if s == 0:
+            e_a = vec_normwise(rg, ro, ("ax", "ay", "az"))
+            print(f"gas A {e_a:.3e}")
+            assert e_a <= 1e-5
