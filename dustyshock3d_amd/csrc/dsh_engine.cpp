@@ -83,6 +83,9 @@ Engine::~Engine() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : allocs_) (void)hipFree(p);
+    for (hipEvent_t e : side_ev_)
+        if (e) (void)hipEventDestroy(e);
+    if (side_stream_) (void)hipStreamDestroy(side_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -228,6 +231,8 @@ int Engine::prepare() {
         C.dt_mode = P.dt_mode;
         C.max_steps = P.max_steps;
         C.precision = P.precision;
+        C.wall_band = P.fp64_wall_band ? 1 : 0;
+        C.band_speed = P.wall_band_speed > 0.0 ? P.wall_band_speed : 0.0;   // 0: a fixed band of 4 cells
         if ((long long)C.ncell * C.nsp > 2000000000ll) return fail(DSH_ERR_ARG, "grid too large");
     }
     // segments: species-major, then tube
@@ -303,6 +308,18 @@ int Engine::prepare() {
     a_.variant = pair_variant_;
     a_.tile = tile_;
     if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
+    a_.pband = nullptr;
+    for (int t = 0; t < T; ++t)
+        if (tc_h_[t].precision == DSH_MIXED && tc_h_[t].wall_band) {
+            if ((rc = alloc(&a_.pband, (size_t)ntot_))) return rc;
+            break;
+        }
+    if (a_.pband) {
+        if ((rc = hip_check(hipStreamCreateWithFlags(&side_stream_, hipStreamNonBlocking), "hipStreamCreate")))
+            return rc;
+        for (hipEvent_t& e : side_ev_)
+            if ((rc = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))) return rc;
+    }
     a_.ginv = nullptr;
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].drag_mode == 0) {
@@ -424,7 +441,7 @@ int Engine::prepare() {
                         return rc;
         }
     double* prep = nullptr;
-    if ((rc = alloc(&prep, 5 * (size_t)a_.pmap.nblocks))) return rc;
+    if ((rc = alloc(&prep, kPrepParts * (size_t)a_.pmap.nblocks))) return rc;
     if ((rc = hip_check(launch_prepare(a_, prep, stream_), "k_prepare"))) return rc;
     if ((rc = hip_check(hipStreamSynchronize(stream_), "prepare sync"))) return rc;
     // adopted tubes continue with their own clock and step size
@@ -444,6 +461,10 @@ int Engine::prepare() {
         d.err = sd.err;
         d.done_cur = 0;
         for (int c = 0; c < 4; ++c) d.cons[c] = sd.cons[c];
+        d.xw_lo = sd.xw_lo;
+        d.xw_hi = sd.xw_hi;
+        d.band_lo = sd.band_lo;
+        d.band_hi = sd.band_hi;
         if ((rc = hip_check(hipMemcpy(a_.td + t, &d, sizeof(TubeDyn), hipMemcpyHostToDevice), "H2D dyn"))) return rc;
         migr_[t].src = nullptr;
     }
@@ -471,6 +492,8 @@ int Engine::enqueue_step(bool timed) {
         (void)hipEventRecord(ev[0], stream_);
         ++pending_timed_;
     }
+    const SidePass side_d{side_stream_, side_ev_[0], side_ev_[1]};
+    const SidePass side_f{side_stream_, side_ev_[2], side_ev_[3]};
     for (int i = 0; i < kStages; ++i) {
         hipError_t e = hipSuccess;
         switch (i) {
@@ -479,8 +502,8 @@ int Engine::enqueue_step(bool timed) {
             case 2: e = launch_scatter(a_, stream_); break;
             case 3: e = launch_rank_permute(a_, stream_); break;
             case 4: e = launch_radius(a_, stream_); break;
-            case 5: e = launch_density(a_, precision_, stats_, stream_); break;
-            case 6: e = launch_force(a_, precision_, stream_); break;
+            case 5: e = launch_density(a_, precision_, stats_, stream_, side_stream_ ? &side_d : nullptr); break;
+            case 6: e = launch_force(a_, precision_, stream_, side_stream_ ? &side_f : nullptr); break;
             case 7: e = launch_idic(a_, stream_); break;
             case 8: e = launch_finalize(a_, stream_); break;
         }

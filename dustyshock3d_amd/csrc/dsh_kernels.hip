@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <type_traits>
 
 namespace dsh {
@@ -103,22 +104,17 @@ template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return 
 // v_sqrt_f32 (1 ulp); arguments are squared distances, never denormal-small in practice.
 template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
-// Plain sum in FP64, Kahan-compensated sum in FP32.
-template <typename R> struct Acc;
-template <> struct Acc<double> {
+// Neighbour sums accumulate in FP64 in both precisions.  An FP32 pair term is exact in FP64
+// and the FP64 sum of a particle's terms rounds far below FP32 resolution, so the exactly
+// antisymmetric FP32 force terms of a pair still cancel in the total momentum; a
+// Kahan-compensated FP32 sum does not (its per-particle result is rounded once more, and
+// those roundings do not cancel between the two particles of a pair).  It also costs fewer
+// VALU slots than Kahan's four FP32 operations: one conversion and one FP64 add, both full
+// rate on CDNA4 without packed FP32.
+template <typename R> struct Acc {
     double s = 0.0;
-    __device__ __forceinline__ void add(double v) { s += v; }
+    __device__ __forceinline__ void add(R v) { s += (double)v; }
     __device__ __forceinline__ double get() const { return s; }
-};
-template <> struct Acc<float> {
-    float s = 0.f, c = 0.f;
-    __device__ __forceinline__ void add(float v) {
-        const float y = v - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double get() const { return (double)s - (double)c; }
 };
 
 // Cubic spline W(r,h) (kernel.cu:11-22) and dW/dr for r >= 0 (kernel.cu:24-38),
@@ -238,11 +234,11 @@ __device__ __forceinline__ ItemPos locate(const Map& mp, int per_block) {
 // partials over every particle, then one block per tube reduces its blocks in order.
 __global__ __launch_bounds__(kBS) void k_prepare_part(Map mp, const TubeConst* __restrict__ tc, PBuf B,
                                                       double* __restrict__ part) {
-    __shared__ double sh[5][kBS / 64];
+    __shared__ double sh[kPrepParts][kBS / 64];
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     const TubeConst& T = tc[ip.sg.tube];
-    double mn = 1e300, px = 0, py = 0, pz = 0, E = 0;
+    double mn = 1e300, px = 0, py = 0, pz = 0, E = 0, flo = -1e300, fhi = 1e300;
     if (li < ip.sg.n) {
         const int p = ip.sg.off + li;
         const double vx = B.vx[p], vy = B.vy[p], vz = B.vz[p], m = B.m[p];
@@ -251,13 +247,22 @@ __global__ __launch_bounds__(kBS) void k_prepare_part(Map mp, const TubeConst* _
         pz = m * vz;
         E = 0.5 * m * (vx * vx + vy * vy + vz * vz);
         if (ip.sg.species == 0) E += m * B.e[p];
-        if (B.ind[p] == 0) mn = B.h[p] / (T.cs + sqrt(vx * vx + vy * vy + vz * vz));
+        if (B.ind[p] == 0) {
+            mn = B.h[p] / (T.cs + sqrt(vx * vx + vy * vy + vz * vz));
+        } else {
+            // frozen wall particle: extent of the wall on its side of the tube's middle
+            const double x = B.x[p];
+            if (x < T.o[0] + 0.5 * T.L[0]) flo = x;
+            else fhi = x;
+        }
     }
     px = wave_sum(px);
     py = wave_sum(py);
     pz = wave_sum(pz);
     E = wave_sum(E);
     mn = wave_min(mn);
+    flo = wave_max(flo);
+    fhi = wave_min(fhi);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         sh[0][w] = px;
@@ -265,49 +270,90 @@ __global__ __launch_bounds__(kBS) void k_prepare_part(Map mp, const TubeConst* _
         sh[2][w] = pz;
         sh[3][w] = E;
         sh[4][w] = mn;
+        sh[5][w] = flo;
+        sh[6][w] = fhi;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a[4] = {0, 0, 0, 0}, m2 = 1e300;
+        double a[4] = {0, 0, 0, 0}, m2 = 1e300, lo = -1e300, hi = 1e300;
         for (int k = 0; k < kBS / 64; ++k) {
             for (int c = 0; c < 4; ++c) a[c] += sh[c][k];
             m2 = fmin(m2, sh[4][k]);
+            lo = fmax(lo, sh[5][k]);
+            hi = fmin(hi, sh[6][k]);
         }
-        double* o = part + 5ll * blockIdx.x;
+        double* o = part + (long long)kPrepParts * blockIdx.x;
         for (int c = 0; c < 4; ++c) o[c] = a[c];
         o[4] = m2;
+        o[5] = lo;
+        o[6] = hi;
     }
+}
+
+// FP64 band of the coming step of a mixed-precision tube.  Momentum enters a tube only
+// through its frozen walls, so the particles next to them, within f * band_speed * cs * (time
+// left) + (3 + f) neighbour cells of the frozen layers (f: neighbour cells per averaging cell
+// along an axis), run in FP64: FP32 rounding made outside the band reaches the walls late or
+// not at all before the run ends.  An open-ended run keeps the fixed part; no band in FP64
+// mode or when switched off.
+__device__ __forceinline__ void set_band(const TubeConst& T, TubeDyn& D) {
+    D.band_lo = -1e300;
+    D.band_hi = 1e300;
+    if (!T.wall_band || T.precision != DSH_MIXED) return;
+    double left = 1e300;
+    if (T.t_end < 1e29) left = T.t_end - D.t;
+    if (T.max_steps > 0) {
+        // a CFL step may grow: twice the current step per step left
+        const double by_steps = (double)(T.max_steps - D.steps) * D.dt * (T.dt_mode == DSH_DT_CFL ? 2.0 : 1.0);
+        left = fmin(left, by_steps);
+    }
+    if (!(left < 1e29)) left = 0.0;   // an open-ended run: the fixed part alone
+    // an averaging cell of f neighbour cells passes a particle's influence f cells on in one
+    // step, so both terms scale with f
+    const double w = T.band_speed * (double)T.f * T.cs * fmax(left, 0.0) + (double)(3 + T.f) * T.c[0];
+    D.band_lo = D.xw_lo + w;
+    D.band_hi = D.xw_hi - w;
 }
 
 __global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __restrict__ tc, TubeDyn* __restrict__ td,
                                                  const double* __restrict__ part) {
-    __shared__ double sh[5][kBS / 64];
+    __shared__ double sh[kPrepParts][kBS / 64];
     const int t = blockIdx.x;
     const TubeConst& T = tc[t];
-    double mn = 1e300, a4[4] = {0, 0, 0, 0};
+    double mn = 1e300, a4[4] = {0, 0, 0, 0}, flo = -1e300, fhi = 1e300;
     for (int s = 0; s < T.nsp; ++s) {
         const int sg = T.pseg[s];
         for (int b = mp.blk[sg] + threadIdx.x; b < mp.blk[sg + 1]; b += kBS) {
-            const double* o = part + 5ll * b;
+            const double* o = part + (long long)kPrepParts * b;
             for (int c = 0; c < 4; ++c) a4[c] += o[c];
             mn = fmin(mn, o[4]);
+            flo = fmax(flo, o[5]);
+            fhi = fmin(fhi, o[6]);
         }
     }
     for (int c = 0; c < 4; ++c) a4[c] = wave_sum(a4[c]);
     mn = wave_min(mn);
+    flo = wave_max(flo);
+    fhi = wave_min(fhi);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         for (int c = 0; c < 4; ++c) sh[c][w] = a4[c];
         sh[4][w] = mn;
+        sh[5][w] = flo;
+        sh[6][w] = fhi;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a[4] = {0, 0, 0, 0}, m2 = 1e300;
+        double a[4] = {0, 0, 0, 0}, m2 = 1e300, lo = -1e300, hi = 1e300;
         for (int k = 0; k < kBS / 64; ++k) {
             for (int c = 0; c < 4; ++c) a[c] += sh[c][k];
             m2 = fmin(m2, sh[4][k]);
+            lo = fmax(lo, sh[5][k]);
+            hi = fmin(hi, sh[6][k]);
         }
         TubeDyn& D = td[t];
+        D.xw_lo = lo;
+        D.xw_hi = hi;
         D.t = 0.0;
         D.dt = (T.dt_mode == 0) ? T.tau : T.cfl * m2;
         D.steps = 0;
@@ -317,6 +363,7 @@ __global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __rest
         D.done_cur = 0;
         for (int c = 0; c < 4; ++c) D.cons[c] = a[c];
         for (int s = 0; s < kMaxSpecies; ++s) D.hmax_bits[s] = 0ull;
+        set_band(T, D);
     }
 }
 
@@ -659,6 +706,62 @@ __global__ __launch_bounds__(kBS) void k_radius(Map mp, const TubeConst* __restr
     reach[c] = (hc + axis_window_max(T, w, wxy, ip.sg.off, li, 2)) * (1.0 + kRadiusMargin);
 }
 
+// Pressure + viscosity terms of one pair (IDIC.h:225-250): returns the acceleration
+// coefficient of (dx,dy,dz) and the de/dt contribution.
+// d2 is the squared distance: FP64 takes sqrt and 1/r; FP32 takes 1/r from v_rsq_f32 and
+// r = d2/r, one transcendental instead of two.
+template <typename T>
+__device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
+                                            T pr2i, T pr2j, T csi, T csj, T mi, T mj, T alpha, T beta, T epsv,
+                                            bool plain, T& coef, T& edc) {
+    T dist, rinv, dist2;
+    if constexpr (sizeof(T) == 8) {
+        dist = sqrt(d2);
+        rinv = rcp<T>(dist);
+        dist2 = dist * dist;
+    } else {
+        rinv = __builtin_amdgcn_rsqf(d2);
+        dist = d2 * rinv;
+        dist2 = d2;
+    }
+    const T ih = rcp<T>(hh);
+    // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
+    const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
+    T fnu = T(0);
+    if (nu < T(0)) {
+        const T cnu = T(0.5) * (csi + csj);
+        const T mu = hh * nu * rcp<T>(dist2 + epsv * hh * hh);
+        fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
+    }
+    const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw_hit<T>(dist * ih);
+    const T dwr = dw * rinv;
+    if (plain)
+        edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
+    else
+        edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
+    coef = mj * (pr2i + pr2j + fnu) * dwr;
+}
+
+// Band flag of every sorted particle's 64-particle chunk (one wave per chunk; the chunks of
+// the tiled pair kernels): 1 if a particle of the chunk lies in its tube's FP64 band this
+// step.  Band chunks run the FP64 pair kernels; FP32 chunks evaluate their pairs with band
+// (and frozen) particles in FP64, so both particles of a pair compute its terms alike.
+__global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict__ td, const int* __restrict__ cur,
+                                              PBuf B, unsigned char* __restrict__ pband) {
+    const ItemPos ip = locate(mp, kBS);
+    const int li = ip.li + threadIdx.x;
+    const bool valid = li < ip.sg.n;
+    const TubeDyn& D = td[ip.sg.tube];
+    const long long wo = (long long)(1 - *cur) * B.ntot;
+    bool in = false;
+    if (valid) {
+        const double x = B.x[wo + ip.sg.off + li];
+        in = x < D.band_lo || x > D.band_hi;
+    }
+    const bool any = __ballot(in) != 0ull;
+    if (valid) pband[ip.sg.off + li] = any ? 1 : 0;
+}
+
 template <typename R, bool kStats>
 struct DensPair {
     const double* __restrict__ M;
@@ -688,7 +791,8 @@ __global__ __launch_bounds__(kBS) void k_density(Map mp, const TubeConst* __rest
                                                  PBuf B, const int* __restrict__ start,
                                                  const int* __restrict__ wkey, const double* __restrict__ reach,
                                                  double* __restrict__ pr2, double* __restrict__ csav,
-                                                 int* __restrict__ ncount, unsigned long long* __restrict__ nhash) {
+                                                 int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
+                                                 const unsigned char* __restrict__ pband) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
@@ -709,19 +813,28 @@ __global__ __launch_bounds__(kBS) void k_density(Map mp, const TubeConst* __rest
         const int kx = lc % T.n[0];
         const int ky = (lc / T.n[0]) % T.n[1];
         const int kz = lc / (T.n[0] * T.n[1]);
-        DensPair<R, kStats> f{B.m + wo, H, B.id + wo, H[q]};
-        for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
-        rho = f.acc.get();
+        // a target of an FP64-band chunk sums in FP64
+        auto run = [&](auto& f) {
+            for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
+            rho = f.acc.get();
+            if (kStats) {
+                ncount[q] = f.cnt;
+                nhash[q] = f.hash;
+            }
+        };
+        if (sizeof(R) == 4 && pband != nullptr && pband[q] != 0) {
+            DensPair<double, kStats> f{B.m + wo, H, B.id + wo, H[q]};
+            run(f);
+        } else {
+            DensPair<R, kStats> f{B.m + wo, H, B.id + wo, H[q]};
+            run(f);
+        }
         B.rho[wo + q] = rho;
         if (s == 0) {
             const double p = (T.eos == 0) ? rho * B.e[wo + q] * (T.gamma - 1) : T.cs * T.cs * rho;
             B.p[wo + q] = p;
         } else {
             if (T.drag_mode == 1) B.ts[wo + q] = rho / T.K[s - 1];
-        }
-        if (kStats) {
-            ncount[q] = f.cnt;
-            nhash[q] = f.hash;
         }
     } else if (kStats) {
         ncount[q] = 0;
@@ -734,6 +847,9 @@ __global__ __launch_bounds__(kBS) void k_density(Map mp, const TubeConst* __rest
     }
 }
 
+// One target's force sums over the direct traversal.  Mixed precision: a pair with a frozen
+// neighbour, or with both particles in FP64-band chunks, is evaluated in FP64 (its partner
+// evaluates it so too), every other pair in FP32, all accumulated in FP64.
 template <typename R>
 struct ForcePair {
     const double* __restrict__ M;
@@ -744,37 +860,35 @@ struct ForcePair {
     const double* __restrict__ RHO;
     const double* __restrict__ PR2;
     const double* __restrict__ CS;
-    double hi, vxi, vyi, vzi;
-    R rhoi, pr2i, csi, mi, alpha, beta, eps;
+    const int* __restrict__ IND;
+    const unsigned char* __restrict__ PB;
+    double hi, vxi, vyi, vzi, rhoi, pr2i, csi, mi, alpha, beta, eps;
     bool plain;   // gas-only run: de/dt of the plain ForceKernel
+    bool f64i;    // the target's chunk is in the FP64 band
     Acc<R> ax, ay, az, ed;
+    double dx_ = 0.0, dy_ = 0.0, dz_ = 0.0, de_ = 0.0;
     __device__ __forceinline__ void pair(int j, double dx, double dy, double dz, double d2, int, int) {
         if (!(d2 > 0.0)) return;
-        const R dist = sqrt_r<R>((R)d2);
-        const R hh = (R)(0.5 * (hi + H[j]));
-        const R ih = rcp<R>(hh);
-        const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
-        // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
-        const R nu = (R)(vxi - VX[j]) * (-fx) + (R)(vyi - VY[j]) * (-fy) + (R)(vzi - VZ[j]) * (-fz);
-        R fnu = R(0);
-        if (nu < R(0)) {
-            const R cnu = R(0.5) * (csi + (R)CS[j]);
-            const R mu = hh * nu * rcp<R>(dist * dist + eps * hh * hh);
-            fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<R>(R(0.5) * (rhoi + (R)RHO[j]));
-        }
-        const R id4 = ih * ih * ih * ih;
-        const R dw = (R)kInvPi * id4 * spline_dw<R>(dist * ih);
-        const R dwr = dw * rcp<R>(dist);
-        if (plain) {
-            const R w = plain_edot_weight<R>((R)(vxi - VX[j]), (R)(vyi - VY[j]), (R)(vzi - VZ[j]), fx, fy, fz, ih);
-            ed.add((mi * pr2i + mi * fnu * R(0.5)) * w);
+        const double hh = 0.5 * (hi + H[j]);
+        if (sizeof(R) == 8 || IND[j] != 0 || (f64i && PB[j] != 0)) {
+            double coef, edc;
+            force_terms<double>(d2, hh, dx, dy, dz, vxi - VX[j], vyi - VY[j], vzi - VZ[j], rhoi, RHO[j], pr2i, PR2[j],
+                                csi, CS[j], mi, M[j], alpha, beta, eps, plain, coef, edc);
+            dx_ += dx * coef;
+            dy_ += dy * coef;
+            dz_ += dz * coef;
+            de_ += edc;
         } else {
-            ed.add((mi * pr2i + mi * fnu * R(0.5)) * nu * dwr);
+            const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
+            R coef, edc;
+            force_terms<R>((R)d2, (R)hh, fx, fy, fz, (R)(vxi - VX[j]), (R)(vyi - VY[j]), (R)(vzi - VZ[j]), (R)rhoi,
+                           (R)RHO[j], (R)pr2i, (R)PR2[j], (R)csi, (R)CS[j], (R)mi, (R)M[j], (R)alpha, (R)beta, (R)eps,
+                           plain, coef, edc);
+            ax.add(fx * coef);
+            ay.add(fy * coef);
+            az.add(fz * coef);
+            ed.add(edc);
         }
-        const R coef = (R)M[j] * (pr2i + (R)PR2[j] + fnu) * dwr;
-        ax.add(fx * coef);
-        ay.add(fy * coef);
-        az.add(fz * coef);
     }
 };
 
@@ -783,7 +897,8 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
                                                const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                                const int* __restrict__ start, const int* __restrict__ wkey,
                                                const double* __restrict__ reach, const double* __restrict__ pr2,
-                                               const double* __restrict__ csav) {
+                                               const double* __restrict__ csav,
+                                               const unsigned char* __restrict__ pband) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
@@ -808,15 +923,15 @@ __global__ __launch_bounds__(kBS) void k_force(Map mp, const TubeConst* __restri
     const int kx = lc % T.n[0];
     const int ky = (lc / T.n[0]) % T.n[1];
     const int kz = lc / (T.n[0] * T.n[1]);
-    ForcePair<R> f{B.m + wo, H, B.vx + wo, B.vy + wo, B.vz + wo, B.rho + wo, pr2, csav,
+    ForcePair<R> f{B.m + wo, H, B.vx + wo, B.vy + wo, B.vz + wo, B.rho + wo, pr2, csav, B.ind + wo, pband,
                    H[q], B.vx[wo + q], B.vy[wo + q], B.vz[wo + q],
-                   (R)B.rho[wo + q], (R)pr2[q], (R)csav[q], (R)B.m[wo + q],
-                   (R)T.alpha, (R)T.beta, (R)T.eps, T.nsp == 1};
+                   B.rho[wo + q], pr2[q], csav[q], B.m[wo + q],
+                   T.alpha, T.beta, T.eps, T.nsp == 1, pband != nullptr && pband[q] != 0};
     for_pairs(T, cell_off, start, kx, ky, kz, reach[wkey[q]], X, Y, Z, H, X[q], Y[q], Z[q], H[q], f);
-    B.ax[wo + q] = f.ax.get();
-    B.ay[wo + q] = f.ay.get();
-    B.az[wo + q] = f.az.get();
-    B.ed[wo + q] = f.ed.get();
+    B.ax[wo + q] = f.ax.get() + f.dx_;
+    B.ay[wo + q] = f.ay.get() + f.dy_;
+    B.az[wo + q] = f.az.get() + f.dz_;
+    B.ed[wo + q] = f.ed.get() + f.de_;
 }
 
 // ---------------------------------------------------------------- tiled pair kernels
@@ -916,42 +1031,6 @@ struct TileLds {
     int id[kStats ? kTileP : 1];
 };
 
-// Pressure + viscosity terms of one pair (IDIC.h:225-250): returns the acceleration
-// coefficient of (dx,dy,dz) and the de/dt contribution.
-// d2 is the squared distance: FP64 takes sqrt and 1/r; FP32 takes 1/r from v_rsq_f32 and
-// r = d2/r, one transcendental instead of two.
-template <typename T>
-__device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx, T dvy, T dvz, T rhoi, T rhoj,
-                                            T pr2i, T pr2j, T csi, T csj, T mi, T mj, T alpha, T beta, T epsv,
-                                            bool plain, T& coef, T& edc) {
-    T dist, rinv, dist2;
-    if constexpr (sizeof(T) == 8) {
-        dist = sqrt(d2);
-        rinv = rcp<T>(dist);
-        dist2 = dist * dist;
-    } else {
-        rinv = __builtin_amdgcn_rsqf(d2);
-        dist = d2 * rinv;
-        dist2 = d2;
-    }
-    const T ih = rcp<T>(hh);
-    // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
-    const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
-    T fnu = T(0);
-    if (nu < T(0)) {
-        const T cnu = T(0.5) * (csi + csj);
-        const T mu = hh * nu * rcp<T>(dist2 + epsv * hh * hh);
-        fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
-    }
-    const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw_hit<T>(dist * ih);
-    const T dwr = dw * rinv;
-    if (plain)
-        edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
-    else
-        edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
-    coef = mj * (pr2i + pr2j + fnu) * dwr;
-}
-
 // KIND 0: density (all species); KIND 1: gas force and de/dt.
 // The gas density pass stores its predicate hit masks (one 64-bit word per lane per
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
@@ -959,18 +1038,22 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
 // Occupancy targets: both passes are latency- as well as VALU-bound, so the register
 // budget is capped at 4 (force) / 5 (density) waves per SIMD; with pieces of 192 candidates
 // the LDS of a wave (9.4 KB FP32 force, 7.7 KB FP32 density) allows as many.
+// One 64-target chunk b (the block index of the 64-per-block map mp) of a tiled pair pass, by
+// one wave.  With pband (mixed precision with an FP64 wall band), a chunk whose band flag does
+// not match the instance's precision is left to the other instance; the force pair of two
+// band particles is FP64, of a frozen and an active particle FP64, of any other two FP32.
 template <typename R, int KIND, bool kStats, int kTileP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(Map mp, const TubeConst* __restrict__ tc,
-                                                    const TubeDyn* __restrict__ td, const int* __restrict__ cur,
-                                                    PBuf B, const int* __restrict__ start,
-                                                    const int* __restrict__ wkey, const double* __restrict__ reach,
-                                                    double* __restrict__ pr2, double* __restrict__ csav,
-                                                    int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
-                                                    unsigned long long* __restrict__ masks, int mask_chunks) {
+__device__ __forceinline__ void pairs_chunk(int b, const Map& mp, const TubeConst* __restrict__ tc,
+                                            const TubeDyn* __restrict__ td, const int* __restrict__ cur,
+                                            const PBuf& B, const int* __restrict__ start,
+                                            const int* __restrict__ wkey, const double* __restrict__ reach,
+                                            double* __restrict__ pr2, double* __restrict__ csav,
+                                            int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
+                                            unsigned long long* __restrict__ masks, int mask_chunks,
+                                            const unsigned char* __restrict__ pband) {
     static_assert(kTileP <= 256, "window lengths are reduced with wave_max_u9");
     static_assert(kTileP >= 64, "a density window of 64 slots fits in one piece");
     __shared__ TileLds<R, KIND, kStats, kTileP> L;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int sgi = find_seg(mp, b);
     const Seg sg = mp.seg[sgi];
     const int lane = threadIdx.x;
@@ -981,6 +1064,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const int s = sg.species;
     const bool valid = li < sg.n;
     const int q = sg.off + (valid ? li : sg.n - 1);
+    // mixed precision with a wall band: the chunks of the band run in the FP64 instance of this
+    // kernel, the others in the FP32 instance (one flag per chunk, so the exit is wave-uniform)
+    if (pband != nullptr && ((uni((int)pband[q]) != 0) != (sizeof(R) == 8))) return;
     const long long wo = (long long)(1 - *cur) * B.ntot;
     const int nx = T.n[0], ny = T.n[1], nz = T.n[2];
     const int cell_off = T.cell_off[s];
@@ -1038,6 +1124,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     };
     const bool far_images =
         kFix && __ballot(rch >= 1.75 * fmin(T.L[1], T.L[2]) || !fix_range_ok(yi, zi)) != 0ull;
+    // an FP64-band wave evaluates its pairs with active neighbours outside the band as their
+    // FP32 waves do, taking dy, dz from FP64 positions under the same rule (a wave whose reach
+    // spans 1.75 box lengths)
+    const bool far_mirror = KIND == 1 && sizeof(R) == 8 && pband != nullptr &&
+                            __ballot(rch >= 1.75 * fmin(T.L[1], T.L[2]) || !fix_range_ok(yi, zi)) != 0ull;
     bool far_piece = far_images;
 
     // One row of cells (a (dy,dz) offset) for the targets of the current x-row: the union
@@ -1093,7 +1184,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
                 L.rho[k] = (R)rho;
                 L.pr2[k] = (R)p2;
                 L.cs[k] = (R)cs;
-                L.ind[k] = ind != 0 ? 1 : 0;
+                // bit 0: frozen (an FP64 pair); bit 1: in an FP64-band chunk
+                L.ind[k] = (unsigned char)((ind != 0 ? 1 : 0) | (pband != nullptr && pband[src] != 0 ? 2 : 0));
             }
             if (kStats) L.id[k] = id;
         }
@@ -1175,8 +1267,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     auto hit = [&](auto shifted_tag, int u, double shy, double shz, int sy, int sz, int gi) {
         constexpr bool kSh = decltype(shifted_tag)::value;
         const double dx = L.x[u] - xi;
+        if constexpr (KIND == 1 && sizeof(R) == 8) {
+            if (pband != nullptr && L.ind[u] == 0) {
+                // FP64-band wave, active neighbour outside the band: the FP32 term exactly as
+                // the neighbour's FP32 wave evaluates its mirror (fixed-point dy, dz, FP32
+                // fields), so the pair stays antisymmetric bit for bit
+                const float fx = (float)dx;
+                float fy, fz, dd;
+                if (!far_mirror) {
+                    fy = (float)fix_diff(fix_coord(L.y[u], T.o[1], ups_y), Yi, sy) * qy;
+                    fz = (float)fix_diff(fix_coord(L.z[u], T.o[2], ups_z), Zi, sz) * qz;
+                    dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+                } else {
+                    const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
+                    const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+                    fy = (float)dy;
+                    fz = (float)dz;
+                    dd = (float)__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+                }
+                if (dd > 0.0f) {
+                    float coef, edc;
+                    force_terms<float>(dd, 0.5f * (hif + (float)L.h[u]), fx, fy, fz, (float)vxi - (float)L.vx[u],
+                                       (float)vyi - (float)L.vy[u], (float)vzi - (float)L.vz[u], (float)rhoi_d,
+                                       (float)L.rho[u], (float)pr2i_d, (float)L.pr2[u], (float)csi_d, (float)L.cs[u],
+                                       (float)mi_d, (float)L.m[u], (float)T.alpha, (float)T.beta, (float)T.eps, plain,
+                                       coef, edc);
+                    a0.add(fx * coef);
+                    a1.add(fy * coef);
+                    a2.add(fz * coef);
+                    a3.add(edc);
+                }
+                return;
+            }
+        }
         if constexpr (kFix) {
-            if (L.ind[u] != 0) {
+            if ((L.ind[u] & 1) != 0) {
                 // a frozen neighbour has no partner term: FP64 keeps the momentum it injects
                 // exact to rounding (its y, z, h and the target's values are read from memory;
                 // the empty asm keeps their addresses from being hoisted out of the loops)
@@ -1242,7 +1367,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
             }
         } else if (dd > 0.0) {
             const double hhd = 0.5 * (hi + L.h[u]);
-            if (sizeof(R) == 4 && L.ind[u] != 0) {
+            if (sizeof(R) == 4 && (L.ind[u] & 1) != 0) {
                 // a frozen neighbour has no partner term: FP64 keeps the momentum it
                 // injects exact to rounding
                 // (the target's FP64 values are read again here rather than kept in registers;
@@ -1520,6 +1645,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
         B.az[wo + q] = active ? a2.get() + d2s : 0.0;
         B.ed[wo + q] = active ? a3.get() + d3 : 0.0;
     }
+}
+
+// Every chunk of the map, one per block (blocks remapped so each XCD takes a contiguous run).
+template <typename R, int KIND, bool kStats, int kTileP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(
+    Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+    const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
+    double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
+    unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
+    const unsigned char* __restrict__ pband) {
+    pairs_chunk<R, KIND, kStats, kTileP>(xcd_remap(blockIdx.x, gridDim.x), mp, tc, td, cur, B, start, wkey, reach, pr2,
+                                         csav, ncount, nhash, masks, mask_chunks, pband);
 }
 
 // kernel.cu:2310-2437 for one component; n_dust 1 or 2, reference operation order.
@@ -1974,6 +2111,7 @@ __device__ void finish_tube(const TubeConst& T, TubeDyn& D, int next_cur, bool f
         D.done = 1;
         D.done_cur = next_cur;
     }
+    set_band(T, D);
 }
 
 // One block per tube: deterministic reduction of the averaging-cell partials, then
@@ -2087,72 +2225,96 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, 0);
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp, a.wtmp + a.ncells, 1);
     hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp + a.ncells, a.reach);
+    // chunk flags of the FP64 wall band (mixed precision), read by both pair passes
+    if (a.pband)
+        hipLaunchKernelGGL(k_band, dim3(a.pmap.nblocks), b, 0, s, a.pmap, a.td, a.cur, a.B, a.pband);
     return hipGetLastError();
 }
 
 template <typename R, int KIND, bool kStats, int kTileP>
-static void launch_tiled_t(const StepArgs& a, const Map& m, hipStream_t s) {
+static void launch_tiled_t(const StepArgs& a, const Map& m, const unsigned char* pband, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
-                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks);
+                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
+                       pband);
 }
 
 template <typename R, int KIND, bool kStats>
-static void launch_tiled(const StepArgs& a, const Map& m, hipStream_t s) {
+static void launch_tiled1(const StepArgs& a, const Map& m, const unsigned char* pband, hipStream_t s) {
     switch (a.tile) {
-        case 128: launch_tiled_t<R, KIND, kStats, 128>(a, m, s); break;
-        case 192: launch_tiled_t<R, KIND, kStats, 192>(a, m, s); break;
-        case 224: launch_tiled_t<R, KIND, kStats, 224>(a, m, s); break;
-        case 256: launch_tiled_t<R, KIND, kStats, 256>(a, m, s); break;
-        default: launch_tiled_t<R, KIND, kStats, 160>(a, m, s); break;
+        case 128: launch_tiled_t<R, KIND, kStats, 128>(a, m, pband, s); break;
+        case 192: launch_tiled_t<R, KIND, kStats, 192>(a, m, pband, s); break;
+        case 224: launch_tiled_t<R, KIND, kStats, 224>(a, m, pband, s); break;
+        case 256: launch_tiled_t<R, KIND, kStats, 256>(a, m, pband, s); break;
+        default: launch_tiled_t<R, KIND, kStats, 160>(a, m, pband, s); break;
     }
 }
 
-hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s) {
+// FP64 mode: the FP64 instance over every chunk.  Mixed: the FP32 instance over the chunks
+// outside the wall band and, with a band, the FP64 instance over the band's chunks; the two
+// write disjoint chunks, so the FP64 one runs on the side stream beside the FP32 one.
+template <int KIND, bool kStats>
+static hipError_t launch_tiled(const StepArgs& a, const Map& m, int precision, hipStream_t s, const SidePass* side) {
+    if (precision == 0) {
+        launch_tiled1<double, KIND, kStats>(a, m, nullptr, s);
+        return hipGetLastError();
+    }
+    if (!a.pband) {
+        launch_tiled1<float, KIND, kStats>(a, m, nullptr, s);
+        return hipGetLastError();
+    }
+    hipStream_t s64 = side ? side->stream : s;
+    hipError_t e;
+    if (side) {
+        if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
+    }
+    launch_tiled1<double, KIND, kStats>(a, m, a.pband, s64);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (side && (e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
+    launch_tiled1<float, KIND, kStats>(a, m, a.pband, s);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (side && (e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s, const SidePass* side) {
     if (a.variant == kPairTiled) {
         if (a.pmap64.nblocks == 0) return hipSuccess;
-        if (precision == 0) {
-            if (stats) launch_tiled<double, 0, true>(a, a.pmap64, s);
-            else launch_tiled<double, 0, false>(a, a.pmap64, s);
-        } else {
-            if (stats) launch_tiled<float, 0, true>(a, a.pmap64, s);
-            else launch_tiled<float, 0, false>(a, a.pmap64, s);
-        }
-        return hipGetLastError();
+        return stats ? launch_tiled<0, true>(a, a.pmap64, precision, s, side)
+                     : launch_tiled<0, false>(a, a.pmap64, precision, s, side);
     }
     const dim3 g(a.pmap.nblocks), b(kBS);
     if (precision == 0) {
         if (stats)
             hipLaunchKernelGGL((k_density<double, true>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey,
-                               a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+                               a.reach, a.pr2, a.csav, a.ncount, a.nhash, nullptr);
         else
             hipLaunchKernelGGL((k_density<double, false>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start,
-                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, nullptr);
     } else {
         if (stats)
             hipLaunchKernelGGL((k_density<float, true>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey,
-                               a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+                               a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.pband);
         else
             hipLaunchKernelGGL((k_density<float, false>), g, b, 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.start,
-                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash);
+                               a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.pband);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
+hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s, const SidePass* side) {
     if (a.variant == kPairTiled) {
         if (a.gmap64.nblocks == 0) return hipSuccess;
-        if (precision == 0) launch_tiled<double, 1, false>(a, a.gmap64, s);
-        else launch_tiled<float, 1, false>(a, a.gmap64, s);
-        return hipGetLastError();
+        return launch_tiled<1, false>(a, a.gmap64, precision, s, side);
     }
     if (a.gmap.nblocks == 0) return hipSuccess;
     const dim3 g(a.gmap.nblocks), b(kBS);
     if (precision == 0)
         hipLaunchKernelGGL((k_force<double>), g, b, 0, s, a.gmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey, a.reach,
-                           a.pr2, a.csav);
+                           a.pr2, a.csav, nullptr);
     else
         hipLaunchKernelGGL((k_force<float>), g, b, 0, s, a.gmap, a.tc, a.td, a.cur, a.B, a.start, a.wkey, a.reach,
-                           a.pr2, a.csav);
+                           a.pr2, a.csav, a.pband);
     return hipGetLastError();
 }
 

@@ -45,6 +45,8 @@ struct TubeConst {
     int drag_mode;         // 0: reference law with the gas particle of the same index, 1: rho_d / K
     double tau, cfl, t_end;
     int eos, h_mode, eps2_quirk, dt_mode, max_steps, precision;
+    int wall_band;         // mixed precision: FP64 band next to the frozen walls (TubeDyn::band_lo/hi)
+    double band_speed;     // band edge speed, units of cs
 };
 
 struct TubeDyn {
@@ -54,6 +56,10 @@ struct TubeDyn {
     int pad_;
     unsigned long long hmax_bits[kMaxSpecies];  // max h per species this step (positive doubles)
     double cons[4];                // momentum (3), energy after the last step
+    // frozen particles lie at x <= xw_lo or x >= xw_hi (x-end walls; +-1e300 if a side has none)
+    double xw_lo, xw_hi;
+    // this step's FP64 band: a 64-particle chunk with a particle at x < band_lo or x > band_hi
+    double band_lo, band_hi;
 };
 
 // Double-buffered particle fields: buffer b of a field starts at base + b * ntot.

@@ -49,7 +49,7 @@ extern "C" {
 
 /* precision */
 #define DSH_FP64 0           /* all arithmetic in FP64 */
-#define DSH_MIXED 1          /* pair sums in FP32 with Kahan compensation; neighbour test,
+#define DSH_MIXED 1          /* pair terms in FP32 accumulated in FP64; neighbour test, wall band,
                                 drag, positions, conservation sums in FP64 */
 
 /*
@@ -76,9 +76,10 @@ typedef struct dsh_params {
     double eta;
     int    n_dust;                /* dust species: 1 or 2; 0 runs gas alone with the plain explicit
                                      scheme of kernel.cu:48-228 */
-    int    drag_mode;             /* 1: t_stop = rho_dust / K (only mode implemented) */
-    double K[DSH_MAX_DUST];
-    double R_dust[DSH_MAX_DUST];  /* reserved for the reference stopping-time law (IDIC.h:78-91) */
+    int    drag_mode;             /* 0: the reference law t_stop = R_dust / (sqrt(gamma p_g / rho_g) rho_g)
+                                     with the gas particle of the same index (IDIC.h:78-91); 1: t_stop = rho_dust / K */
+    double K[DSH_MAX_DUST];       /* drag_mode 1 */
+    double R_dust[DSH_MAX_DUST];  /* drag_mode 0 (R of t_stop_Kernel_IDIC, IDIC.h:78-91) */
     double mass_ratio[DSH_MAX_DUST];   /* m_dust / m_gas: eps = n_d mu / n_g (kernel.cu:2192-2211) */
     double eps_fallback[DSH_MAX_DUST]; /* 0.5, 0.33 in the reference (kernel.cu:2200,2209) */
     double tstop_fallback;             /* 1.0 in the reference (kernel.cu:2160,2188) */
@@ -90,6 +91,18 @@ typedef struct dsh_params {
     double cell_slack;
     int    avg_factor;            /* averaging cell = avg_factor^3 neighbour cells (1 for the shock tubes) */
     int    precision;             /* DSH_FP64 or DSH_MIXED */
+    /* DSH_MIXED only.  A tube's momentum changes only through the force of its frozen x-end
+       walls: the FP32 pair terms of two active particles are antisymmetric bit for bit and
+       accumulate in FP64, so they cancel (exactly for equal masses).  With fp64_wall_band = 1
+       the particles within f * wall_band_speed * cs * (time left) + (3 + f) neighbour cells of
+       the frozen layers (f = avg_factor) run their density and force in FP64 (per 64-particle
+       chunk; a pair is FP64
+       when both particles are in the band or one is frozen), so the wall force, and with it
+       the total momentum, follows an FP64 run: to 1e-13 per step, and over a run until the
+       first particle is assigned to another averaging cell than in the FP64 run.  The band's
+       time term needs a finite run (t_end < 1e29 or max_steps > 0); without one, 3 + f cells. */
+    int    fp64_wall_band;        /* 1 (default) or 0 */
+    double wall_band_speed;       /* band edge speed in units of cs (default 0.5) */
 } dsh_params;
 
 /* data.h structure-of-arrays view; NULL fields are skipped. */

@@ -941,6 +941,8 @@ int orc_tube_params(const dsh_tube_spec* t, dsh_params* p) {
     p->cell_slack = 0.05;
     p->avg_factor = 1;
     p->precision = t->precision;
+    p->fp64_wall_band = 1;
+    p->wall_band_speed = 0.5;
     return DSH_OK;
 }
 

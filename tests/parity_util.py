@@ -52,3 +52,19 @@ def momentum_error(cons_a, cons_b, states_b: dict):
         scale += float(np.sum(arrs["m"] * np.sqrt(arrs["vx"] ** 2 + arrs["vy"] ** 2 + arrs["vz"] ** 2)))
     scale = max(float(np.linalg.norm(P)), scale, 1e-300)
     return float(np.linalg.norm(np.asarray(cons_a[:3]) - P)) / scale
+
+
+def energy_error(cons_a, cons_b):
+    """|dE| / |E| of the total energy sum (kinetic + internal), the fourth conservation entry."""
+    return abs(float(cons_a[3]) - float(cons_b[3])) / max(abs(float(cons_b[3])), 1e-300)
+
+
+def quantile_normwise(a: dict, b: dict, q: float, keys=("vx", "vy", "vz"), mask=None):
+    """q-quantile over particles of max_k |a_k - b_k| / max |b| (a robust companion of vec_normwise)."""
+    da = np.stack([a[k] for k in keys], 1)
+    db = np.stack([b[k] for k in keys], 1)
+    if mask is not None:
+        da, db = da[mask], db[mask]
+    s = float(np.max(np.abs(db))) if db.size else 0.0
+    e = np.max(np.abs(da - db), axis=1) / (s if s > 0 else 1.0)
+    return float(np.quantile(e, q)) if e.size else 0.0

@@ -45,9 +45,10 @@ def test_cfg2_run_to_t_end_is_near_the_coupled_limit():
 def test_cfg2_full_size_step_parity(oracle_mod, warm_steps):
     """One step of the full 2 x 2,812,500-particle tube from a shared state (the set-up, or the
     GPU's state after 20 steps): keys, sorted order and neighbour sets bit-exact against the
-    oracle; mixed-precision fields within the cfg0 parity tolerances."""
+    oracle; mixed-precision fields within the cfg0 parity tolerances; momentum <= 1e-9 and
+    energy <= 1e-8 against the oracle."""
     from dustyshock3d_amd.abi import MIXED
-    from tests.parity_util import inject, take_state, rel_pointwise, vec_normwise
+    from tests.parity_util import energy_error, inject, momentum_error, take_state, rel_pointwise, vec_normwise
 
     spec = configs.cfg2()
     g0 = Simulation.from_spec(spec)
@@ -77,5 +78,11 @@ def test_cfg2_full_size_step_parity(oracle_mod, warm_steps):
         print(f"species {s}: neighbours {int(cg.sum())} rho {e_rho:.3e} v {e_v:.3e}", flush=True)
         assert e_rho <= 1e-5
         assert e_v <= 1e-5
+    st_o = take_state(o)
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    e_e = energy_error(g.conservation(), o.conservation())
+    print(f"momentum {e_p:.3e} energy {e_e:.3e}", flush=True)
+    assert e_p <= 1e-9
+    assert e_e <= 1e-8
     so, sg = o.status(), g.status()
     assert abs(sg["dt"] - so["dt"]) <= 1e-6 * so["dt"]
