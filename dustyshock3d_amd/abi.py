@@ -185,6 +185,7 @@ def _declare(lib):
         "dsh_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),
         "dsh_last_error": (C.c_char_p, [vp]),
         "dsh_destroy": (None, [vp]),
+        "dsh_step_noalloc": (C.c_int, [P(Params), P(SoaView), _IP, C.c_int, P(SoaView), _DP, _DP, vp]),
         "dsh_batch_create": (C.c_int, [P(Params), C.c_int, P(vp)]),
         "dsh_batch_from_specs": (C.c_int, [P(TubeSpec), C.c_int, P(vp)]),
         "dsh_batch_upload": (C.c_int, [vp, C.c_int, C.c_int, P(SoaView), C.c_int]),

@@ -45,6 +45,35 @@ void build_grid(const dsh_params& P, double hmin, TubeConst& T) {
     T.navg = T.na[0] * T.na[1] * T.na[2];
 }
 
+void tube_physics(const dsh_params& P, TubeConst& C) {
+    C.nsp = 1 + P.n_dust;
+    C.gamma = P.gamma;
+    C.cs = P.cs;
+    C.alpha = P.alpha;
+    C.beta = P.beta;
+    C.eps = P.eps;
+    C.eta = P.eta;
+    for (int k = 0; k < kMaxDust; ++k) {
+        C.K[k] = P.K[k];
+        C.mu[k] = P.mass_ratio[k];
+        C.eps_fb[k] = P.eps_fallback[k];
+        C.R[k] = P.R_dust[k];
+    }
+    C.drag_mode = P.drag_mode;
+    C.t_fb = P.tstop_fallback;
+    C.tau = P.tau;
+    C.cfl = P.cfl;
+    C.t_end = P.t_end;
+    C.eos = P.eos_type;
+    C.h_mode = P.h_mode;
+    C.eps2_quirk = P.eps2_uses_count1;
+    C.dt_mode = P.dt_mode;
+    C.max_steps = P.max_steps;
+    C.precision = P.precision;
+    C.wall_band = P.fp64_wall_band ? 1 : 0;
+    C.band_speed = P.wall_band_speed > 0.0 ? P.wall_band_speed : 0.0;   // 0: the fixed part of the band alone
+}
+
 Engine::Engine(const std::vector<dsh_params>& params)
     : params_(params), staged_(params.size() * kMaxSpecies), migr_(params.size()) {}
 
@@ -207,32 +236,7 @@ int Engine::prepare() {
             }
             build_grid(P, hmin, C);
         }
-        C.nsp = 1 + P.n_dust;
-        C.gamma = P.gamma;
-        C.cs = P.cs;
-        C.alpha = P.alpha;
-        C.beta = P.beta;
-        C.eps = P.eps;
-        C.eta = P.eta;
-        for (int k = 0; k < kMaxDust; ++k) {
-            C.K[k] = P.K[k];
-            C.mu[k] = P.mass_ratio[k];
-            C.eps_fb[k] = P.eps_fallback[k];
-            C.R[k] = P.R_dust[k];
-        }
-        C.drag_mode = P.drag_mode;
-        C.t_fb = P.tstop_fallback;
-        C.tau = P.tau;
-        C.cfl = P.cfl;
-        C.t_end = P.t_end;
-        C.eos = P.eos_type;
-        C.h_mode = P.h_mode;
-        C.eps2_quirk = P.eps2_uses_count1;
-        C.dt_mode = P.dt_mode;
-        C.max_steps = P.max_steps;
-        C.precision = P.precision;
-        C.wall_band = P.fp64_wall_band ? 1 : 0;
-        C.band_speed = P.wall_band_speed > 0.0 ? P.wall_band_speed : 0.0;   // 0: a fixed band of 4 cells
+        tube_physics(P, C);
         if ((long long)C.ncell * C.nsp > 2000000000ll) return fail(DSH_ERR_ARG, "grid too large");
     }
     // segments: species-major, then tube

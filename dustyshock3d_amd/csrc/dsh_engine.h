@@ -23,6 +23,8 @@ struct HostSpecies {
 
 // Grid rule shared by every entry point (the oracle restates it independently).
 void build_grid(const dsh_params& P, double hmin, TubeConst& T);
+// Physics, time-step and precision fields of a tube's constants from its parameters.
+void tube_physics(const dsh_params& P, TubeConst& C);
 
 class Engine {
 public:

@@ -206,6 +206,22 @@ int dsh_kernel_times(dsh_ctx* ctx, double* ms, char* names, int max_kernels, int
 const char* dsh_last_error(dsh_ctx* ctx);
 void dsh_destroy(dsh_ctx* ctx);
 
+/* No counterpart in the reference: the variant of dsh_step (with dsh_create,
+   dsh_upload_species and dsh_download_species around it) for callers that cannot allocate
+   device memory, such as code under HIP graph capture or tight serving loops.  One tube of
+   counts[s] particles per species s is read from the caller's DEVICE arrays in[s] (data.h
+   layout, original index order; x, y, z, m, h, ind required, other fields 0 when NULL),
+   stepped up to nsteps steps from t = 0 with the kernels of dsh_step (tiled pair kernels,
+   piece size 192), and written to the caller's DEVICE arrays out[s] in original order (nn
+   must be NULL; cell as dsh_download_species).  *t_out and cons_out (optional) receive the
+   time and the conservation sums.  The step's scratch (sorted state, keys, cell arrays, hit
+   masks, partials) comes from one static 64 MiB array per GPU, not from an allocation: a tube
+   that does not fit (roughly 2 x 40,000 particles on a few thousand cells) is rejected with
+   DSH_ERR_BUDGET before anything runs.  Runs on `stream` (a hipStream_t, NULL for the null
+   stream) and waits for it at the end.  Calls that overlap on one GPU share its array and race. */
+int dsh_step_noalloc(const dsh_params* params, const dsh_soa_view* in, const int* counts, int nsteps,
+                     const dsh_soa_view* out, double* t_out, double cons_out[4], void* stream);
+
 /* ---- batch of independent tubes (no reference counterpart: the reference runs one
    problem per process; the batch runs many in one launch sequence per step) ---- */
 typedef struct dsh_batch dsh_batch;
