@@ -1,0 +1,245 @@
+"""The benchmark configurations against the FP64 oracle on the GPU, beyond cfg0 and cfg2.
+
+- Stiff drag (the regime the FP64 implicit IDIC solve is for, kernel.cu:2310-2437,
+  IDIC.h:264-315): cfg0-geometry tubes with K = 1000 (dt / t_stop 1.8 to 14) and K = 1e5
+  (180 to 1440).  FP64: 50 steps within 1e-12.  Mixed: every step up to the first particle
+  that sits in another averaging cell than in the oracle's run (the cell is a truncated FP64
+  divide, kernel.cu:2071; a particle within rounding distance of a face lands on either side,
+  and its cell means jump), with momentum <= 1e-9 and fields <= 1e-5; that horizon must reach
+  20 steps.
+- cfg1 at full size (2 x 703,125 particles, K = 1000, mixed): one step from a shared state,
+  sets bit-exact, fields <= 1e-5, momentum <= 1e-9, energy <= 1e-8.
+- cfg3 extremes (left spacing 0.0025 / 0.01, K 0.01 / 1000, dust-to-gas 0.01 / 10): all eight
+  corners in one batch to t = 0.2 in FP64, four of them (every extreme of every parameter)
+  against the oracle's own run to t = 0.2; and the cheap corners in mixed precision, batched,
+  in lockstep with the oracle up to the first averaging-cell flip, and one step from the
+  batch's own states at three points of the run with momentum <= 1e-9.
+- cfg4: a Poisson stream of requests (sweep tubes and one cfg1-sized tube) through the
+  continuous-batching server; every result equals its solo run bit for bit.
+"""
+import numpy as np
+import pytest
+
+from dustyshock3d_amd import Simulation, abi, configs
+from dustyshock3d_amd.batch import Batch, Server
+from tests.parity_util import (energy_error, inject, momentum_error, rel_pointwise, take_state, vec_normwise)
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "rho", "h", "e", "t_stop", "ax", "ay", "az")
+
+
+def _fields_error(got: dict, ref: dict) -> float:
+    e = 0.0
+    for s in ref:
+        act = ref[s]["ind"] == 0
+        e = max(e, rel_pointwise(got[s]["rho"][act], ref[s]["rho"][act]), vec_normwise(got[s], ref[s]))
+    return e
+
+
+def _flipped(got: dict, ref: dict) -> bool:
+    return any(not np.array_equal(got[s]["cell"], ref[s]["cell"]) for s in ref)
+
+
+def _lockstep(oracle_mod, make_gpu, specs, steps, momentum=True):
+    """GPU tubes and their oracle runs one step at a time; returns, per tube, the first step with
+    an averaging-cell flip (or None) after checking every step before it (fields <= 1e-5 and,
+    with momentum, momentum <= 1e-9)."""
+    oracles = [oracle_mod.OracleSim.from_spec(s) for s in specs]
+    step_gpu, download, conservation = make_gpu
+    first = [None] * len(specs)
+    for k in range(1, steps + 1):
+        step_gpu()
+        for i, o in enumerate(oracles):
+            if first[i] is not None or o.status()["done"]:
+                continue
+            o.step(1)
+            st_o = take_state(o)
+            st_g = {s: download(i, s) for s in (0, 1)}
+            if _flipped(st_g, st_o):
+                first[i] = k
+                print(f"tube {i}: first averaging-cell flip at step {k}", flush=True)
+                continue
+            e_p = momentum_error(conservation(i), o.conservation(), st_o)
+            e_f = _fields_error(st_g, st_o)
+            assert e_f <= 1e-5 and (e_p <= 1e-9 or not momentum), (i, k, e_p, e_f)
+    return first
+
+
+def _shared_step(oracle_mod, params, state, label):
+    """One GPU step and one oracle step from the same state: sets bit-exact, fields <= 1e-5,
+    momentum <= 1e-9, energy <= 1e-8."""
+    o = inject(oracle_mod.OracleSim, params, state)
+    g = inject(Simulation, params, state)
+    g.set_stats(True)
+    o.step(1)
+    g.step(1)
+    st_o = take_state(o)
+    st_g = {s: g.download(s) for s in (0, 1)}
+    for s in (0, 1):
+        assert np.array_equal(st_g[s]["cell"], st_o[s]["cell"]) and np.array_equal(st_g[s]["nn"], st_o[s]["nn"])
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co) and np.array_equal(hg, ho), (label, s)
+    e_f = _fields_error(st_g, st_o)
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    e_e = energy_error(g.conservation(), o.conservation())
+    print(f"{label}: one step from the GPU state: fields {e_f:.2e} momentum {e_p:.2e} energy {e_e:.2e}", flush=True)
+    assert e_f <= 1e-5 and e_p <= 1e-9 and e_e <= 1e-8, label
+
+
+@pytest.mark.parametrize("K", [1000.0, 1e5])
+def test_stiff_drag_fp64_against_oracle(oracle_mod, K):
+    spec = configs.tube(0.05, 0.005, K, 1.0, max_steps=50, precision=abi.FP64)
+    o = oracle_mod.OracleSim.from_spec(spec)
+    g = Simulation.from_spec(spec)
+    d = o.download(1)
+    ratio = o.status()["dt"] / d["t_stop"][d["ind"] == 0]
+    print(f"\nstiff K={K}: dt / t_stop from {ratio.min():.1f} to {ratio.max():.1f}", flush=True)
+    assert ratio.min() >= 1.5 if K == 1000.0 else ratio.min() >= 150.0
+    o.step(50)
+    g.step(50)
+    st_o = take_state(o)
+    st_g = {s: g.download(s) for s in (0, 1)}
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    e_e = energy_error(g.conservation(), o.conservation())
+    e_f = _fields_error(st_g, st_o)
+    print(f"after 50 steps: fields {e_f:.2e} momentum {e_p:.2e} energy {e_e:.2e}")
+    assert e_f <= 1e-12 and e_p <= 1e-12 and e_e <= 1e-12
+
+
+@pytest.mark.parametrize("K", [1000.0, 1e5])
+def test_stiff_drag_mixed_lockstep(oracle_mod, K):
+    spec = configs.tube(0.05, 0.005, K, 1.0, max_steps=50, precision=abi.MIXED)
+    g = Simulation.from_spec(spec)
+    print(f"\nstiff mixed K={K}", flush=True)
+    first = _lockstep(oracle_mod, (lambda: g.step(1), lambda i, s: g.download(s), lambda i: g.conservation()),
+                      [spec], 50)
+    print(f"first averaging-cell flip: {first[0]}")
+    assert first[0] is None or first[0] > 20
+
+
+@pytest.mark.parametrize("warm_steps", [0, 20])
+def test_cfg1_full_size_step_parity(oracle_mod, warm_steps):
+    spec = configs.cfg1()
+    g0 = Simulation.from_spec(spec)
+    params = g0.params
+    if warm_steps:
+        g0.step(warm_steps)
+    state = take_state(g0)
+    del g0
+    print(f"\ncfg1 full-size parity warm={warm_steps}", flush=True)
+    o = inject(oracle_mod.OracleSim, params, state)
+    g = inject(Simulation, params, state)
+    g.set_stats(True)
+    o.step(1)
+    g.step(1)
+    st_o = take_state(o)
+    for s in (0, 1):
+        ro, rg = st_o[s], g.download(s)
+        assert np.array_equal(rg["cell"], ro["cell"]) and np.array_equal(rg["nn"], ro["nn"])
+        co, ho = o.neighbour_stats(s)
+        cg, hg = g.neighbour_stats(s)
+        assert np.array_equal(cg, co) and np.array_equal(hg, ho), f"species {s}: neighbour sets differ"
+        act = ro["ind"] == 0
+        e_rho = rel_pointwise(rg["rho"][act], ro["rho"][act])
+        e_v = vec_normwise(rg, ro)
+        print(f"species {s}: neighbours {int(cg.sum())} rho {e_rho:.3e} v {e_v:.3e}", flush=True)
+        assert e_rho <= 1e-5 and e_v <= 1e-5
+        if s == 1:
+            assert rel_pointwise(rg["t_stop"][act], ro["t_stop"][act]) <= 1e-5
+    e_p = momentum_error(g.conservation(), o.conservation(), st_o)
+    e_e = energy_error(g.conservation(), o.conservation())
+    print(f"momentum {e_p:.3e} energy {e_e:.3e}")
+    assert e_p <= 1e-9 and e_e <= 1e-8
+
+
+CORNERS = [(dx, K, d2g) for dx in (0.0025, 0.01) for K in (0.01, 1000.0) for d2g in (0.01, 10.0)]
+# every extreme of every parameter, checked against the oracle's run to t = 0.2
+ORACLE_CORNERS = (1, 2, 4, 7)
+
+
+def _corner_specs(precision):
+    return [configs.tube(0.05, dx, K, d2g, t_end=0.2, dt_mode=abi.DT_CFL, precision=precision, tube_index=i)
+            for i, (dx, K, d2g) in enumerate(CORNERS)]
+
+
+def test_sweep_extremes_fp64_to_t_end(oracle_mod):
+    specs = _corner_specs(abi.FP64)
+    print(f"\ncfg3 extremes, FP64 batch of {len(specs)} tubes to t = 0.2", flush=True)
+    b = Batch(specs)
+    b.run(100000)
+    for i in ORACLE_CORNERS:
+        o = oracle_mod.OracleSim.from_spec(specs[i])
+        o.step(100000)
+        so, sb = o.status(), b.status(i)
+        assert sb["done"] and so["done"] and sb["steps"] == so["steps"] and sb["t"] == so["t"] == 0.2
+        st_o = take_state(o)
+        st_b = {s: b.download(i, s) for s in (0, 1)}
+        e_f = _fields_error(st_b, st_o)
+        e_p = momentum_error(b.conservation(i), o.conservation(), st_o)
+        e_e = energy_error(b.conservation(i), o.conservation())
+        print(f"tube {CORNERS[i]}: {sb['steps']} steps, fields {e_f:.2e} momentum {e_p:.2e} energy {e_e:.2e}",
+              flush=True)
+        assert e_f <= 1e-5 and e_p <= 1e-9 and e_e <= 1e-9
+
+
+def test_sweep_extremes_mixed(oracle_mod):
+    """The cheap corners (left spacing 0.01, covering the extremes of K and dust-to-gas) in mixed
+    precision, batched.  These coarse tubes hold one or two particles per species per averaging
+    cell, so most cells take the reference's fallback (no dust or no gas: eps 0.5, t 1 against
+    a dust at rest, kernel.cu:2160-2211), whose drag removes momentum in proportion to the local
+    gas velocity: FP32-level velocity differences anywhere then reach the total momentum.  So:
+    fields <= 1e-5 every step up to the first averaging-cell flip (how soon one comes depends on
+    how many particles sit near cell faces), and from the batch's own states at steps 1, 15 and
+    30, one step against the oracle with momentum <= 1e-9."""
+    specs = [s for s, c in zip(_corner_specs(abi.MIXED), CORNERS) if c[0] == 0.01]
+    print(f"\ncfg3 extremes, mixed batch of {len(specs)} tubes", flush=True)
+    b = Batch(specs)
+    first = _lockstep(oracle_mod, (lambda: b.run(1), lambda i, s: b.download(i, s), lambda i: b.conservation(i)),
+                      specs, 30, momentum=False)
+    print(f"first averaging-cell flips: {first}")
+    b2 = Batch(specs)
+    done = 0
+    for k in (1, 15, 30):
+        b2.run(k - done)
+        done = k
+        for i, spec in enumerate(specs):
+            state = {s: b2.download(i, s) for s in (0, 1)}
+            from dustyshock3d_amd.sim import tube_params
+            _shared_step(oracle_mod, tube_params(spec), state, f"tube {i} step {k}")
+
+
+def test_serving_stream_matches_solo_runs():
+    """A Poisson stream of nine sweep tubes and one cfg1-sized tube (each to t = 0.2) through the
+    server, admitted as they arrive into the running batch: every result is bit-identical to
+    the same request run alone."""
+    rng = np.random.default_rng(11)
+    reqs = [configs.sweep_tube(20000 + i, t_end=0.2) for i in range(9)]
+    reqs.insert(3, configs.large_tube(5, t_end=0.2))
+    arrivals = np.cumsum(rng.exponential(0.02, size=len(reqs)))
+    clock = {"t": 0.0}
+    sv = Server(hbm_budget=200e9, steps_per_epoch=8, clock=lambda: clock["t"])
+    ids, k = [], 0
+    print(f"\nserving stream: {len(reqs)} requests, one large", flush=True)
+    for _ in range(100000):
+        while k < len(reqs) and arrivals[k] <= clock["t"]:
+            ids.append(sv.submit(reqs[k], float(arrivals[k])))
+            k += 1
+        r, q = sv.pump()
+        clock["t"] += 0.01
+        if k == len(reqs) and r == 0 and q == 0:
+            break
+    st = sv.stats()
+    print(f"rebuilds {st['rebuilds']} epochs {st['epochs']}", flush=True)
+    assert st["rebuilds"] >= 2
+    for rid, spec in zip(ids, reqs):
+        assert sv.poll(rid)["state"] == abi.REQ_DONE
+        s = Simulation.from_spec(spec)
+        s.step(100000)
+        assert s.status()["t"] == 0.2
+        for sp in (0, 1):
+            rv, rs = sv.result(rid, sp), s.download(sp)
+            for f in FIELDS:
+                assert np.array_equal(rv[f], rs[f]), (rid, sp, f)
+        s.close()
