@@ -1,15 +1,19 @@
 #!/usr/bin/env python3
-"""Benchmark: particle-steps/s of the dusty-gas SPH step on the 2 x 2,812,500-particle tube.
+"""Benchmark: particle-steps/s of the dusty-gas SPH step on the 2 x 2,812,500-particle tube,
+with the cfg3 sweep and the cfg4 serving stream in the same JSON line.
 
 Workload (cfg2): isothermal dusty shock tube, x in [-0.5,0.5], y,z in [0,0.2) periodic,
 lattice spacing 0.002 / 0.004, density 1 / 0.125, dust-to-gas 1, K = 100, seeded jitter,
-CFL 0.3 step, mixed precision (FP32 pair sums with Kahan compensation; FP64 drag,
-positions and conservation sums).  Inputs are resident in HBM before the timed region.
+CFL 0.3 steps to t = 0.2, mixed precision (FP32 pair terms accumulated in FP64, an FP64 band
+next to the frozen walls; FP64 drag, positions and conservation sums).  Inputs are resident
+in HBM before the timed region.
 
 A "step" is one pass of the whole per-step pipeline (sort, density, force, IDIC drag +
-kick-drift, per-tube reductions) over both species.  Every stage is bracketed by HIP
-events on the step stream; the dominant stage's average duration gives the roofline
-figure with the algorithmic bytes of its pass.
+kick-drift, per-tube reductions) over both species.  The K timed steps are spread evenly over
+the run to t = 0.2 (h and the band width change along it), each bracketed by device
+synchronisations; every stage is bracketed by HIP events on the step stream, and the dominant
+stage's average duration gives the roofline figure with the algorithmic bytes of its pass.
+The whole run, graph-replayed, and the same run in FP64 are reported beside it.
 """
 from __future__ import annotations
 
@@ -105,6 +109,16 @@ def _oracle_rate(spec, threads, steps):
 
 
 def _cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -114,40 +128,76 @@ def _cpu_model():
     return None
 
 
-def cpu_baseline(spec, threads, steps=1):
-    """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube; and on one
-    thread, one step of the cfg1-sized tube (a quarter of the particles)."""
+def cpu_threads():
+    """The host threads this process may use: OMP_NUM_THREADS when set (the share of the host
+    that goes with one GPU), else the cores the process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(spec, threads, steps=3):
+    """The FP64 OpenMP oracle on the host cores: `steps` steps of the same tube from its set-up
+    (threads bound one per core, OMP_PROC_BIND=close, OMP_PLACES=cores); and on one thread, one
+    step of the cfg1-sized tube (a quarter of the particles)."""
     from dustyshock3d_amd import configs
 
     rate, n, dt, cores = _oracle_rate(spec, threads, steps)
     small = configs.cfg1()
     rate1, n1, dt1, _ = _oracle_rate(small, 1, 1)
     return {"value": rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} step(s) of the full {n}-particle tube (first step from the initial state), "
-                      f"FP64 OpenMP oracle, {dt:.1f} s",
+            "sample": f"{steps} steps of the full {n}-particle tube from its set-up, FP64 OpenMP oracle, "
+                      f"{dt:.1f} s",
+            "binding": {k: os.environ.get(k) for k in ("OMP_PROC_BIND", "OMP_PLACES", "OMP_NUM_THREADS")},
             "one_thread": {"value": rate1, "sample": f"1 step of the {n1}-particle cfg1 tube, {dt1:.1f} s"},
-            "cpu_model": _cpu_model()}
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def whole_run(spec, precision=None):
+    """The tube from its set-up to t_end, graph-replayed, untimed by stage: (steps, seconds)."""
+    from dustyshock3d_amd import Simulation
+
+    if precision is not None:
+        spec.precision = precision
+    s = Simulation.from_spec(spec)
+    s.step(0)
+    t0 = time.perf_counter()
+    s.step(1 << 20)
+    dt = time.perf_counter() - t0
+    steps = s.status()["steps"]
+    s.close()
+    return steps, dt
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="cfg2", choices=["cfg0", "cfg1", "cfg2"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
     ap.add_argument("--no-autotune", action="store_true", help=f"keep the default piece size ({DEFAULT_TILE})")
+    ap.add_argument("--no-sweep", action="store_true", help="leave out the cfg3 sweep object")
+    ap.add_argument("--no-serve", action="store_true", help="leave out the cfg4 serving object")
+    ap.add_argument("--no-whole-run", action="store_true", help="leave out the whole-run figures")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    # the oracle's OpenMP threads (CPU baseline): one per core, close together
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
-    from dustyshock3d_amd import Simulation, configs
+    from dustyshock3d_amd import Simulation, abi, configs
 
-    spec = getattr(configs, args.config)(max_steps=0)
+    spec = getattr(configs, args.config)()
     t_setup = time.perf_counter()
     sim = Simulation.from_spec(spec)
     sim.set_pair_variant(args.variant)
@@ -159,16 +209,33 @@ def main():
     sim.step(args.warmup)
     setup_s = time.perf_counter() - t_setup
     st0 = sim.status()
-    sim.set_timing(True)
-    t0 = time.perf_counter()
-    sim.step(args.steps)
-    wall = time.perf_counter() - t0
+    # the K timed steps, one every `stride` steps of the rest of the run; sim.step returns after
+    # the device has finished, so each timed step is bracketed by synchronisations
+    end_t = spec.t_end if spec.t_end < 1e29 else None
+    left = (end_t - st0["t"]) / st0["dt"] if end_t else float(args.steps)
+    stride = max(1, int(0.9 * left / max(args.steps, 1)))
+    wall, stage_sum, steps_done, t_samples = 0.0, {}, 0, []
+    for _ in range(args.steps):
+        if stride > 1:
+            sim.set_timing(False)
+            sim.step(stride - 1)
+        if sim.status()["done"]:
+            break
+        sim.set_timing(True)
+        t_samples.append(sim.status()["t"])
+        t0 = time.perf_counter()
+        sim.step(1)
+        wall += time.perf_counter() - t0
+        steps_done += 1
+        for k, v in sim.kernel_times().items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
+    sim.set_timing(False)
     st1 = sim.status()
-    kt = sim.kernel_times()
-    steps_done = st1["steps"] - st0["steps"]
+    kt = {k: v / max(steps_done, 1) for k, v in stage_sum.items()}
     if steps_done != args.steps:
         print(f"warning: only {steps_done} of {args.steps} steps ran (tube finished)", file=sys.stderr)
     n_avg = sim.grid()["n_avg"]
+    sim.close()
     value = (n_gas + n_dust) * steps_done / wall
 
     groups = {"sort": sum(kt.get(k, 0.0) for k in SORT_STAGES)}
@@ -187,13 +254,26 @@ def main():
                         if groups[k] > 0 else None,
                         "hbm_frac": round(b / (groups[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if groups[k] > 0 else None}
 
+    runs = None
+    if not args.no_whole_run and end_t:
+        n_mix, s_mix = whole_run(getattr(configs, args.config)())
+        n_64, s_64 = whole_run(getattr(configs, args.config)(), abi.FP64)
+        runs = {"mixed": {"steps": n_mix, "seconds": round(s_mix, 3), "ms_per_step": round(s_mix * 1e3 / n_mix, 4),
+                          "particle_steps_per_s": (n_gas + n_dust) * n_mix / s_mix},
+                "fp64": {"steps": n_64, "seconds": round(s_64, 3), "ms_per_step": round(s_64 * 1e3 / n_64, 4),
+                         "particle_steps_per_s": (n_gas + n_dust) * n_64 / s_64},
+                "note": "set-up to t = 0.2 as one graph-replayed run, no stage events"}
+
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_threads()
         cpu = cpu_baseline(spec, threads, args.cpu_steps)
 
+    sweep = None if args.no_sweep else sweep_result(SweepArgs())
+    serve = None if args.no_serve else serve_result(ServeArgs())
+
     out = {
-        "metric": "particle-steps/s at 2x2.8M (HBM roofline %)",
+        "metric": "particle-steps/s at 2x2.8M (HBM roofline %); sweep p50/p99 latency vs serial",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -203,12 +283,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions" if spec.precision == 1 else "f64",
+        "dtype": "f32 pair terms accumulated in f64 + f64 wall band, drag, positions, reductions"
+                 if spec.precision == 1 else "f64",
         "data": "synthetic (seeded lattice shock tube)",
         "config": {"workload": f"{args.config}: dusty shock tube, {n_gas} gas + {n_dust} dust particles, "
-                               f"y,z width {spec.y_width}, dx {spec.dx_left}/{2 * spec.dx_left}, K={spec.K}, CFL 0.3",
-                   "particles": n_gas + n_dust, "averaging_cells": n_avg, "t_at_start": st0["t"],
-                   "dt": st1["dt"], "setup_s": round(setup_s, 2),
+                               f"y,z width {spec.y_width}, dx {spec.dx_left}/{2 * spec.dx_left}, K={spec.K}, CFL 0.3"
+                               + (f", to t = {end_t}" if end_t else ""),
+                   "particles": n_gas + n_dust, "averaging_cells": n_avg,
+                   "timed_steps": f"{steps_done} single steps, one every {stride} steps of the run, "
+                                  f"t = {t_samples[0]:.4f} to {t_samples[-1]:.4f}" if t_samples else None,
+                   "t_after": st1["t"], "setup_s": round(setup_s, 2),
                    "pair_kernels": "tiled" if args.variant == 1 else "direct",
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -225,7 +309,10 @@ def main():
                           "frac": step_bytes / (wall / max(steps_done, 1)) / 1e9 / HBM_PEAK_GBS},
         "stages_ms": {k: round(v, 4) for k, v in kt.items()},
         "passes": per_stage,
+        "whole_run": runs,
         "cpu_baseline": cpu,
+        "sweep": sweep,
+        "serve": serve,
     }
     print(json.dumps(out))
 
@@ -245,7 +332,17 @@ def run_alone(spec, variant=1):
     return sum(s.counts.values()) * st["steps"], dt
 
 
-def sweep_mode(args):
+class SweepArgs:
+    """Defaults of the cfg3 sweep as the main benchmark line runs it."""
+    variant, tubes, chunk, serial_stride = 1, 512, 16, 8
+
+
+class ServeArgs:
+    """Defaults of the cfg4 stream as the main benchmark line runs it."""
+    variant, requests, load, sat_rate, budget, epoch, seed = 1, 60, 0.8, 0.0, 200e9, 8, 7
+
+
+def sweep_result(args) -> dict:
     """cfg3: 512 independent tubes in one batch, each run to t=0.2, against the same tubes
     run one at a time (a sample of every `serial_stride`-th tube) on the same GPU."""
     from dustyshock3d_amd import configs
@@ -277,7 +374,7 @@ def sweep_mode(args):
         sdt += d
     serial_rate = sps / sdt
     tts = np.array(sorted(finish.values()))
-    out = {
+    return {
         "metric": "sweep particle-steps/s (cfg3, 512 tubes in one batch)",
         "value": ps / wall, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
         "vs_serial_gpu": (ps / wall) / serial_rate,
@@ -286,15 +383,18 @@ def sweep_mode(args):
         "wall_s": wall, "setup_s": setup, "particle_steps": ps,
         "tubes": len(specs), "steps_min_max": [min(steps), max(steps)],
         "time_to_solution_s": {"p50": float(np.percentile(tts, 50)), "p99": float(np.percentile(tts, 99))},
-        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
+        "dtype": "f32 pair terms accumulated in f64 + f64 wall band, drag, positions, reductions", "data": "synthetic",
         "config": {"workload": "cfg3 sweep: y,z in [0,0.05], dx log-uniform [0.0025,0.01], K log-uniform "
                                "[0.01,1000], dust-to-gas uniform [0.01,10], each to t=0.2",
                    "pair_kernels": "tiled" if args.variant == 1 else "direct"},
     }
-    print(json.dumps(out))
 
 
-def serve_mode(args):
+def sweep_mode(args):
+    print(json.dumps(sweep_result(args)))
+
+
+def serve_result(args) -> dict:
     """cfg4: Poisson arrivals at a fraction of the measured saturated throughput, 95% sweep tubes
     and 5% large tubes, continuous batching under an HBM budget; the same request stream replayed
     one request at a time (FIFO, measured service times) gives the serial comparison."""
@@ -348,7 +448,7 @@ def serve_mode(args):
     tts = np.array([p["finished"] - p["arrival"] for p in done])
     ps = sum(p["particle_steps"] for p in done)
     st = sv.stats()
-    out = {
+    return {
         "metric": "serving particle-steps/s and time-to-solution (cfg4 stream, continuous batching)",
         "value": ps / wall, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
         "requests": len(reqs), "completed": len(done),
@@ -361,11 +461,14 @@ def serve_mode(args):
         "serial_particle_steps_per_s": float(psteps.sum() / service.sum()),
         "wall_s": wall, "rebuilds": st["rebuilds"], "epochs": st["epochs"],
         "rebuild_s": st["rebuild_s"], "advance_s": st["advance_s"], "hbm_budget": args.budget,
-        "dtype": "f32 pair sums (Kahan) + f64 drag/positions/reductions", "data": "synthetic",
+        "dtype": "f32 pair terms accumulated in f64 + f64 wall band, drag, positions, reductions", "data": "synthetic",
         "config": {"workload": "cfg4: Poisson arrivals, 95% cfg3-distribution tubes, 5% large tubes "
                                "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch},
     }
-    print(json.dumps(out))
+
+
+def serve_mode(args):
+    print(json.dumps(serve_result(args)))
 
 
 def copy_mode(args):
