@@ -121,18 +121,37 @@ def empty_arrays(n: int, fields=ALL_FIELDS) -> dict:
     return out
 
 
+def field_arrays(arrays: dict, fields, n: int = None) -> dict:
+    """Contiguous int32 / float64 copies (or the arrays themselves) of the given fields that are
+    present, each checked to be one-dimensional with n entries (n: the length of x).  The C ABI
+    reads n entries of every field it is given, so a shorter array would be read past its end:
+    a length mismatch raises ValueError instead."""
+    if n is None:
+        if "x" not in arrays or arrays["x"] is None:
+            raise ValueError("the arrays need an x field")
+        n = len(arrays["x"])
+    out = {}
+    for f in fields:
+        a = arrays.get(f)
+        if a is None:
+            continue
+        a = np.ascontiguousarray(a, dtype=np.int32 if f in INT_FIELDS else np.float64)
+        if a.ndim != 1 or a.shape[0] != n:
+            raise ValueError(f"field {f!r} has shape {a.shape}, expected ({n},)")
+        out[f] = a
+    return out
+
+
 def make_view(arrays: dict) -> SoaView:
     """Build a view over numpy arrays (the arrays must stay alive while the view is used)."""
     v = SoaView()
     for f, a in arrays.items():
         if a is None:
             continue
-        if f in INT_FIELDS:
-            assert a.dtype == np.int32 and a.flags.c_contiguous, f
-            setattr(v, f, a.ctypes.data_as(_IP))
-        else:
-            assert a.dtype == np.float64 and a.flags.c_contiguous, f
-            setattr(v, f, a.ctypes.data_as(_DP))
+        want = np.int32 if f in INT_FIELDS else np.float64
+        if a.dtype != want or not a.flags.c_contiguous:
+            raise TypeError(f"field {f!r} must be a contiguous {np.dtype(want).name} array")
+        setattr(v, f, a.ctypes.data_as(_IP if f in INT_FIELDS else _DP))
     return v
 
 
@@ -181,6 +200,7 @@ def _declare(lib):
         "dsh_batch_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_double)]),
         "dsh_set_pair_variant": (C.c_int, [vp, C.c_int]),
         "dsh_neighbour_stats": (C.c_int, [vp, C.c_int, _IP, P(C.c_uint64)]),
+        "dsh_sorted_ids": (C.c_int, [vp, C.c_int, _IP]),
         "dsh_set_timing": (C.c_int, [vp, C.c_int]),
         "dsh_kernel_times": (C.c_int, [vp, _DP, C.c_char_p, C.c_int, C.c_int]),
         "dsh_last_error": (C.c_char_p, [vp]),

@@ -122,6 +122,11 @@ int dsh_neighbour_stats(dsh_ctx* c, int species, int* count, uint64_t* hash) {
     return c->set(c->eng->neighbour_stats(0, species, count, hash));
 }
 
+int dsh_sorted_ids(dsh_ctx* c, int species, int* ids) {
+    if (!c) return DSH_ERR_ARG;
+    return c->set(c->eng->sorted_ids(0, species, ids));
+}
+
 int dsh_set_timing(dsh_ctx* c, int on) {
     if (!c) return DSH_ERR_ARG;
     c->eng->set_timing(on != 0);

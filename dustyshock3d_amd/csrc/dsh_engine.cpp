@@ -759,6 +759,19 @@ int Engine::download(int tube, int species, const dsh_soa_view* v) {
     return DSH_OK;
 }
 
+int Engine::sorted_ids(int tube, int species, int* ids) {
+    if (tube < 0 || tube >= ntubes() || species < 0 || !ids) return fail(DSH_ERR_ARG, "bad tube/species");
+    if (!prepared_ || dyn_h_.empty() || species >= tc_h_[tube].nsp) return fail(DSH_ERR_STATE, "no step yet");
+    int rc;
+    if ((rc = hip_check(hipStreamSynchronize(stream_), "sorted ids sync"))) return rc;
+    int cur = 0;
+    if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
+    // the current buffer holds the last step's particles in that step's sorted order
+    return hip_check(hipMemcpy(ids, a_.B.id + (size_t)cur * ntot_ + tc_h_[tube].p_off[species],
+                               sizeof(int) * (size_t)tc_h_[tube].np[species], hipMemcpyDeviceToHost),
+                     "D2H sorted ids");
+}
+
 int Engine::neighbour_stats(int tube, int species, int* count, uint64_t* hash) {
     if (tube < 0 || tube >= ntubes() || species < 0 || species >= tc_h_[tube].nsp)
         return fail(DSH_ERR_ARG, "bad tube/species");

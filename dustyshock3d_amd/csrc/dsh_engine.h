@@ -50,6 +50,8 @@ public:
     int sync();
     int download(int tube, int species, const dsh_soa_view* v);
     int neighbour_stats(int tube, int species, int* count, uint64_t* hash);
+    // particle indices of one (tube, species) in the device's sorted order of the last step
+    int sorted_ids(int tube, int species, int* ids);
     int refresh_dyn();   // copy per-tube dynamic state to the host
     const TubeDyn& dyn(int tube) const { return dyn_h_[tube]; }
     const TubeConst& tube_const(int tube) const { return tc_h_[tube]; }

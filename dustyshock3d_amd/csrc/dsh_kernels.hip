@@ -66,8 +66,11 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// Order-free fingerprint term of one neighbour: (id, image) packs injectively into 64 bits
+// (image offsets are at most kMaxCellReach = 64 < 128 cells away, so sy + 128 and sz + 128 fit
+// 8 bits each) and mix64 is a bijection, so distinct neighbours give distinct terms.
 __device__ __forceinline__ uint64_t pair_tag(int id, int sy, int sz) {
-    return mix64(((uint64_t)(uint32_t)id << 8) ^ ((uint64_t)(sy + 8) << 4) ^ (uint64_t)(sz + 8));
+    return mix64(((uint64_t)(uint32_t)id << 16) | ((uint64_t)(uint32_t)(sy + 128) << 8) | (uint64_t)(uint32_t)(sz + 128));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

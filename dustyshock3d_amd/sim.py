@@ -73,7 +73,7 @@ def reference_ic(p: Params, species: int) -> dict:
 def write_data(directory: str, num: int, t: float, tau: float, h: float, species: int, arrays: dict):
     """Data_out / Data_out_dust files of one species (kernel.cu:232-826)."""
     n = len(arrays["x"])
-    v = make_view(arrays)
+    v = make_view(abi.field_arrays(arrays, abi.ALL_FIELDS, n))
     _check(abi.lib().dsh_write_data(directory.encode(), num, t, tau, h, species, C.byref(v), n - 1),
            "write_data")
 
@@ -81,6 +81,8 @@ def write_data(directory: str, num: int, t: float, tau: float, h: float, species
 def reference_nn(p: Params, x, y, z) -> np.ndarray:
     """Next-particle pointers of the reference's lists on its 2h grid."""
     x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    if not (x.ndim == y.ndim == z.ndim == 1 and len(x) == len(y) == len(z)):
+        raise ValueError("x, y and z must be one-dimensional and of equal length")
     nn = np.empty(len(x), dtype=np.int32)
     dp = C.POINTER(C.c_double)
     _check(abi.lib().dsh_reference_nn(C.byref(p), x.ctypes.data_as(dp), y.ctypes.data_as(dp),
@@ -119,8 +121,7 @@ class Simulation:
 
     def upload(self, species: int, arrays: dict):
         n = len(arrays["x"])
-        keep = {f: np.ascontiguousarray(arrays[f], dtype=np.int32 if f == "ind" else np.float64)
-                for f in UPLOAD_FIELDS if f in arrays and arrays[f] is not None}
+        keep = abi.field_arrays(arrays, UPLOAD_FIELDS, n)
         v = make_view(keep)
         self._call(self._lib.dsh_upload_species(self._ctx, species, C.byref(v), n - 1))
         self.counts[species] = n
@@ -179,6 +180,12 @@ class Simulation:
         self._call(self._lib.dsh_neighbour_stats(self._ctx, species, cnt.ctypes.data_as(C.POINTER(C.c_int)),
                                                  h.ctypes.data_as(C.POINTER(C.c_uint64))))
         return cnt, h
+
+    def sorted_ids(self, species: int) -> np.ndarray:
+        """Particle indices in the device's sorted order of the last step."""
+        out = np.zeros(self.counts[species], np.int32)
+        self._call(self._lib.dsh_sorted_ids(self._ctx, species, out.ctypes.data_as(C.POINTER(C.c_int))))
+        return out
 
     def set_timing(self, on: bool = True):
         self._call(self._lib.dsh_set_timing(self._ctx, int(on)))

@@ -200,6 +200,10 @@ int dsh_set_tile(dsh_ctx* ctx, int tile);
    and its time per step.  The trial steps are ordinary steps of the run. */
 int dsh_autotune(dsh_ctx* ctx, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step);
 int dsh_neighbour_stats(dsh_ctx* ctx, int species, int* count, uint64_t* hash);
+/* Particle indices in the device's sorted order of the last step: by neighbour cell, inside a
+   cell by x at the start of the step, then by index (the reference's lists, kernel.cu:1937-1995,
+   are the same cells in index order; dsh_download_species rebuilds them as nn). */
+int dsh_sorted_ids(dsh_ctx* ctx, int species, int* ids);
 /* Per-kernel GPU time of the last timed step, ms (HIP events on the step stream). */
 int dsh_set_timing(dsh_ctx* ctx, int on);
 int dsh_kernel_times(dsh_ctx* ctx, double* ms, char* names, int max_kernels, int name_len);

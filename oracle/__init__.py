@@ -12,7 +12,7 @@ import subprocess
 
 import numpy as np
 
-from dustyshock3d_amd.abi import (ALL_FIELDS, OK, DshError, Params, SoaView, TubeSpec, empty_arrays,
+from dustyshock3d_amd.abi import (ALL_FIELDS, OK, DshError, Params, SoaView, TubeSpec, empty_arrays, field_arrays,
                                   make_view)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -147,8 +147,7 @@ class OracleSim:
 
     def upload(self, species: int, arrays: dict):
         n = len(arrays["x"])
-        keep = {f: np.ascontiguousarray(arrays[f], dtype=np.int32 if f == "ind" else np.float64)
-                for f in UPLOAD_FIELDS if f in arrays and arrays[f] is not None}
+        keep = field_arrays(arrays, UPLOAD_FIELDS, n)
         v = make_view(keep)
         rc = self._lib.orc_upload(self._h, species, C.byref(v), n - 1)
         if rc != OK:

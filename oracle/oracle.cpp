@@ -77,8 +77,9 @@ uint64_t mix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
+// (id, image) packed injectively (image offsets within +-127), then the bijective mix64
 uint64_t pair_tag(int id, int sy, int sz) {
-    return mix64(((uint64_t)(uint32_t)id << 8) ^ ((uint64_t)(sy + 8) << 4) ^ (uint64_t)(sz + 8));
+    return mix64(((uint64_t)(uint32_t)id << 16) | ((uint64_t)(uint32_t)(sy + 128) << 8) | (uint64_t)(uint32_t)(sz + 128));
 }
 
 struct Grid {

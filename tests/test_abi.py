@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from dustyshock3d_amd import abi
 
@@ -50,3 +51,33 @@ def test_errors_on_bad_arguments():
     assert lib.dsh_tube_count(C.byref(spec), n) == abi.ERR_ARG   # zero spacing
     assert lib.dsh_read_init(b"/nonexistent/Init.txt", C.byref(abi.Params())) == abi.ERR_IO
     assert np.isfinite(0.0)
+
+
+def test_field_length_mismatch_is_rejected(oracle_mod, tmp_path):
+    """Every field handed to the C ABI must hold n entries (n = len(x)); a shorter one would be
+    read past its end, so the host layer raises ValueError before any call."""
+    from dustyshock3d_amd import abi, configs
+    from dustyshock3d_amd.sim import write_data, reference_nn, tube_params
+
+    spec = configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=2)
+    ic = oracle_mod.tube_ic(spec, 0)
+    n = len(ic["x"])
+    ok = abi.field_arrays(ic, abi.ALL_FIELDS)
+    assert set(ok) <= set(abi.ALL_FIELDS) and all(len(a) == n for a in ok.values())
+    for field in ("vx", "h", "ind"):
+        bad = dict(ic)
+        bad[field] = ic[field][:-1]
+        with pytest.raises(ValueError, match=field):
+            abi.field_arrays(bad, abi.ALL_FIELDS)
+        with pytest.raises(ValueError, match=field):
+            oracle_mod.OracleSim(oracle_mod.tube_params(spec)).upload(0, bad)
+        with pytest.raises(ValueError, match=field):
+            write_data(str(tmp_path), 0, 0.0, 1e-3, 0.01, 0, bad)
+    two_d = dict(ic)
+    two_d["m"] = np.stack([ic["m"], ic["m"]], 1)
+    with pytest.raises(ValueError, match="m"):
+        abi.field_arrays(two_d, abi.ALL_FIELDS)
+    with pytest.raises(ValueError):
+        reference_nn(tube_params(spec), ic["x"], ic["y"][:-1], ic["z"])
+    with pytest.raises(TypeError):
+        abi.make_view({"x": np.zeros(3, np.float32)})

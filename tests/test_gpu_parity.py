@@ -8,9 +8,10 @@ order-free hash of (id, periodic image) per particle).  Fields within tolerance:
   momentum    |dP| / max(|P|, sum m|v|) <= 1e-9 in both modes, also after 50 mixed steps.
               A tube's momentum changes only through its frozen walls: the FP32 pair terms
               between active particles are antisymmetric bit for bit and accumulate in FP64,
-              so they cancel; pairs with a frozen or FP64-band particle are FP64; and the FP64
-              wall band covers every particle whose FP32 rounding could reach a wall before the
-              run ends (2 cs x time left + 4 cells), so the wall force follows the FP64 run.
+              so they cancel; pairs with a frozen particle, or with both particles in the FP64
+              wall band, are FP64; and the band covers every particle whose FP32 rounding could
+              reach a wall before the run ends (f x 0.5 cs x time left + (3 + f) cells, f the
+              averaging factor), so the wall force follows the FP64 run.
   energy      |dE| / |E| <= 1e-12 (FP64), <= 1e-8 (mixed).
 Over long mixed runs a particle can land on the other side of an averaging-cell face than in
 the FP64 run (the cell is a truncated FP64 divide, kernel.cu:2071); its cell means, and so its
@@ -275,3 +276,26 @@ def test_dense_cells_parity(oracle_mod):
             e_a = vec_normwise(rg, ro, ("ax", "ay", "az"))
             print(f"gas A {e_a:.3e}")
             assert e_a <= 1e-5
+
+
+@pytest.mark.parametrize("spec_name", ["cfg0", "corner"])
+def test_device_in_cell_order(oracle_mod, spec_name):
+    """The device sorts by (cell, x at the start of the step, id): cells in the oracle's key
+    order, and inside a cell the order of a lexicographic sort, so the rows of cells the pair
+    kernels scan are x-sorted runs.  The reference's lists (kernel.cu:1937-1995) hold the same
+    cells in index order, which the rebuilt nn lists match bit for bit."""
+    spec = configs.cfg0() if spec_name == "cfg0" else configs.tube(0.04, 0.01, 1.0, 0.1, max_steps=6)
+    params, state = _oracle_state(oracle_mod, spec, 3)
+    print(f"\nin-cell order {spec_name}", flush=True)
+    o, g = _one_step_both(oracle_mod, params, state, MIXED)
+    for s in (0, 1):
+        key = g.download(s)["cell"]
+        ids = g.sorted_ids(s)
+        n = len(key)
+        assert np.array_equal(np.sort(ids), np.arange(n))
+        expect = np.lexsort((np.arange(n), state[s]["x"], key))
+        assert np.array_equal(ids, expect), f"species {s}: in-cell order differs"
+        # the same cells in the same sequence as the oracle's (key, id) order
+        assert np.array_equal(key[ids], key[o.sorted_ids(s)])
+        # cells hold several particles here, so the x order is not the index order
+        assert not np.array_equal(ids, o.sorted_ids(s))

@@ -30,7 +30,9 @@ def mix64(x):
 
 def pair_tag(ids, sy, sz):
     ids = np.asarray(ids, dtype=np.uint64)
-    return mix64((ids << np.uint64(8)) ^ (np.uint64(sy + 8) << np.uint64(4)) ^ np.uint64(sz + 8))
+    sy = np.asarray(np.asarray(sy) + 128, dtype=np.uint64)
+    sz = np.asarray(np.asarray(sz) + 128, dtype=np.uint64)
+    return mix64((ids << np.uint64(16)) | (sy << np.uint64(8)) | sz)
 
 
 def exact_pred(dx, dy, dz, hs):
@@ -134,3 +136,13 @@ def test_thin_tube_uses_periodic_images(small_step):
     cnt_img, _, _ = brute_force(state[0], L, imgs=3)
     cnt_min, _, _ = brute_force(state[0], L, imgs=0)
     assert np.any(cnt_img > cnt_min)
+
+
+def test_pair_tag_is_injective_over_ids_and_images():
+    """The neighbour-set hash sums one tag per (id, periodic image): distinct neighbours must
+    give distinct tags for every image offset the search can reach (at most 64 cells)."""
+    ids = np.array([0, 1, 2, 255, 256, 4095, 4096, 65535, 65536, 2812499, 2 ** 31 - 1])
+    off = np.arange(-64, 65)
+    I, SY, SZ = np.meshgrid(ids, off, off, indexing="ij")
+    tags = pair_tag(I.ravel(), SY.ravel(), SZ.ravel())
+    assert len(np.unique(tags)) == tags.size

@@ -124,3 +124,153 @@ def test_lattice_setup(oracle_mod):
     frozen = g["ind"] == 1
     assert np.all((g["x"][frozen] < -0.5 + 2.5 * 0.006 + 1e-9) | (g["x"][frozen] > 0.5 - 2.5 * 0.012 - 1e-9))
     assert frozen.sum() > 0
+
+
+# --- Monaghan viscosity (IDIC.h:213-235): properties that fix alpha, beta, c_bar, eps and signs ---
+#   mu = hh (v_i - v_j).(r_i - r_j) / (d^2 + eps hh^2)  for approaching pairs, else no viscosity
+#   Pi = (-alpha c_bar mu + beta mu^2) / rho_bar,  c_bar = (c_i + c_j)/2,  c = sqrt(gamma p / rho)
+# The reference takes gamma into the viscosity sound speed also for the isothermal law
+# (c = sqrt(gamma) cs there); the tests below fix that, the linear / quadratic split and the
+# placement of eps, so that a dropped factor, a swapped sign or eps on the wrong term fails.
+
+def _gas_accel(oracle_mod, params, st, **over):
+    p = oracle_mod.Params.from_buffer_copy(params)
+    for k, v in over.items():
+        setattr(p, k, v)
+    o = oracle_mod.OracleSim(p)
+    for s in st:
+        o.upload(s, {k: st[s][k] for k in STATE_FIELDS})
+    o.step(1)
+    g = o.download(0)
+    return np.stack([g["ax"], g["ay"], g["az"]], 1)
+
+
+def _nw(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _blob_params(oracle_mod, spec):
+    params = oracle_mod.tube_params(spec)
+    params.x_min, params.x_max = -0.3, 0.3
+    params.eos_type = 1
+    return params
+
+
+def test_viscosity_is_affine_in_alpha_beta_and_scales_with_velocity(oracle_mod):
+    spec, st = _blob_state(oracle_mod, seed=5)
+    params = _blob_params(oracle_mod, spec)
+    acc = lambda a, b, state=st: _gas_accel(oracle_mod, params, state, alpha=a, beta=b)
+    A00, A10, A01 = acc(0.0, 0.0), acc(1.0, 0.0), acc(0.0, 1.0)
+    Aa, Ab = A10 - A00, A01 - A00
+    assert np.max(np.abs(Aa)) > 0 and np.max(np.abs(Ab)) > 0
+    # affine: no cross term, each coefficient enters once
+    A = acc(0.7, 1.9)
+    assert _nw(A - A00, 0.7 * Aa + 1.9 * Ab) < 1e-11
+    # velocities x lam (lam > 0 keeps the approaching pairs): pressure part unchanged bit for
+    # bit, alpha part x lam, beta part x lam^2
+    lam = 2.5
+    sl = {s: dict(st[s]) for s in st}
+    for k in ("vx", "vy", "vz"):
+        sl[0][k] = st[0][k] * lam
+    B00 = acc(0.0, 0.0, sl)
+    assert np.array_equal(B00, A00)
+    assert _nw(acc(1.0, 0.0, sl) - B00, lam * Aa) < 1e-11
+    assert _nw(acc(0.0, 1.0, sl) - B00, lam * lam * Ab) < 1e-11
+
+
+def test_viscosity_sound_speed_takes_gamma_for_the_isothermal_law(oracle_mod):
+    spec, st = _blob_state(oracle_mod, seed=6)
+    params = _blob_params(oracle_mod, spec)
+    alpha_part = lambda gam: (_gas_accel(oracle_mod, params, st, alpha=1.0, beta=0.0, gamma=gam)
+                              - _gas_accel(oracle_mod, params, st, alpha=0.0, beta=0.0, gamma=gam))
+    assert _nw(alpha_part(1.4), np.sqrt(1.4) * alpha_part(1.0)) < 1e-11
+    # the beta part has no sound speed
+    beta_part = lambda gam: (_gas_accel(oracle_mod, params, st, alpha=0.0, beta=1.0, gamma=gam)
+                             - _gas_accel(oracle_mod, params, st, alpha=0.0, beta=0.0, gamma=gam))
+    assert _nw(beta_part(1.4), beta_part(1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("H", [1.5, -1.5])
+def test_viscosity_acts_on_approaching_pairs_only(oracle_mod, H):
+    """vx = H x, vy = vz = 0: every pair has (v_i - v_j).(r_i - r_j) = H dx^2, so an expanding
+    flow (H > 0) has no viscosity at all and a contracting one (H < 0) has it on every pair
+    with dx != 0.  A flipped sign in mu swaps the two."""
+    spec, st = _blob_state(oracle_mod, seed=7)
+    params = _blob_params(oracle_mod, spec)
+    st[0]["vx"] = H * st[0]["x"]
+    st[0]["vy"][:] = 0.0
+    st[0]["vz"][:] = 0.0
+    A0 = _gas_accel(oracle_mod, params, st, alpha=0.0, beta=0.0)
+    A1 = _gas_accel(oracle_mod, params, st, alpha=1.0, beta=2.0)
+    if H > 0:
+        assert np.array_equal(A1, A0)
+    else:
+        assert _nw(A1, A0) > 1e-3
+        # viscosity dissipates: its work on the flow is negative
+        v = np.stack([st[0]["vx"], st[0]["vy"], st[0]["vz"]], 1)
+        assert np.sum(st[0]["m"] * np.sum(v * (A1 - A0), 1)) < 0
+
+
+def _pair_state(d, h, u, e=(1.0, 1.0)):
+    """Two gas particles on the x axis, d apart, closing at 2u."""
+    a = {k: np.zeros(2) for k in STATE_FIELDS}
+    a["ind"] = np.zeros(2, np.int32)
+    a["x"][:] = (-0.5 * d, 0.5 * d)
+    a["vx"][:] = (u, -u)
+    a["m"][:] = 1e-6
+    a["h"][:] = h
+    a["e"][:] = e
+    a["rho"][:] = 1.0
+    a["p"][:] = 1.0
+    return {0: a}
+
+
+def _pair_params(oracle_mod, eos_type, gamma):
+    params = oracle_mod.tube_params(configs.tube(0.06, 0.01, 1.0, 1.0, max_steps=5))
+    params.n_dust = 0
+    params.x_min, params.x_max = -0.05, 0.05
+    params.eos_type = eos_type
+    params.gamma = gamma
+    params.h_mode = 0
+    return params
+
+
+@pytest.mark.parametrize("d", [0.004, 0.013, 0.019])
+def test_viscosity_of_one_pair_closed_form(oracle_mod, d):
+    """One approaching pair (no periodic image within reach, width 0.06 > 2 (h_i + h_j)):
+    mu = hh (v_i - v_j).(r_i - r_j) / (d^2 + eps hh^2) = -2 u d h / (d^2 + eps h^2), so
+      alpha part / beta part = -alpha c_bar / (beta mu)        (independent of W and rho_bar)
+      alpha part (eps1) / alpha part (eps2) = (d^2 + eps2 h^2) / (d^2 + eps1 h^2)."""
+    h, u, gam = 0.01, 0.3, 1.4
+    params = _pair_params(oracle_mod, 1, gam)
+    st = _pair_state(d, h, u)
+    part = lambda **kw: (_gas_accel(oracle_mod, params, st, **kw)
+                         - _gas_accel(oracle_mod, params, st, alpha=0.0, beta=0.0, eps=kw["eps"]))
+    for eps in (0.01, 1.0):
+        Aa = part(alpha=1.0, beta=0.0, eps=eps)[:, 0]
+        Ab = part(alpha=0.0, beta=1.0, eps=eps)[:, 0]
+        mu = -2.0 * u * d * h / (d * d + eps * h * h)
+        cbar = np.sqrt(gam) * params.cs
+        assert np.all(Aa != 0)
+        assert np.allclose(Aa / Ab, -cbar / mu, rtol=1e-12)
+        # equal and opposite on the two particles
+        assert Aa[0] == pytest.approx(-Aa[1], rel=1e-14)
+        # the pair decelerates: particle 0 moves right and is pushed left
+        assert Aa[0] < 0
+    r = part(alpha=1.0, beta=0.0, eps=0.01)[:, 0] / part(alpha=1.0, beta=0.0, eps=1.0)[:, 0]
+    assert np.allclose(r, (d * d + 1.0 * h * h) / (d * d + 0.01 * h * h), rtol=1e-12)
+
+
+def test_viscosity_sound_speed_is_the_pair_mean(oracle_mod):
+    """Adiabatic law, c = sqrt(gamma (gamma - 1) e): the alpha part follows (c_i + c_j) / 2."""
+    d, h, u, gam = 0.012, 0.01, 0.3, 5.0 / 3.0
+    params = _pair_params(oracle_mod, 0, gam)
+
+    def alpha_part(e):
+        st = _pair_state(d, h, u, e)
+        return (_gas_accel(oracle_mod, params, st, alpha=1.0, beta=0.0)
+                - _gas_accel(oracle_mod, params, st, alpha=0.0, beta=0.0))[:, 0]
+
+    c = lambda e: np.sqrt(gam * (gam - 1.0) * e)
+    same, mixed = alpha_part((1.0, 1.0)), alpha_part((1.0, 4.0))
+    assert np.allclose(mixed / same, 0.5 * (c(1.0) + c(4.0)) / c(1.0), rtol=1e-12)
