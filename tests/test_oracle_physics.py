@@ -133,7 +133,8 @@ def test_lattice_setup(oracle_mod):
 # (c = sqrt(gamma) cs there); the tests below fix that, the linear / quadratic split and the
 # placement of eps, so that a dropped factor, a swapped sign or eps on the wrong term fails.
 
-def _gas_accel(oracle_mod, params, st, **over):
+def _gas_rates(oracle_mod, params, st, **over):
+    """Gas acceleration (n, 3) and de/dt of the force pass of one step."""
     p = oracle_mod.Params.from_buffer_copy(params)
     for k, v in over.items():
         setattr(p, k, v)
@@ -142,7 +143,11 @@ def _gas_accel(oracle_mod, params, st, **over):
         o.upload(s, {k: st[s][k] for k in STATE_FIELDS})
     o.step(1)
     g = o.download(0)
-    return np.stack([g["ax"], g["ay"], g["az"]], 1)
+    return np.stack([g["ax"], g["ay"], g["az"]], 1), g["edot"]
+
+
+def _gas_accel(oracle_mod, params, st, **over):
+    return _gas_rates(oracle_mod, params, st, **over)[0]
 
 
 def _nw(a, b):
@@ -206,9 +211,15 @@ def test_viscosity_acts_on_approaching_pairs_only(oracle_mod, H):
         assert np.array_equal(A1, A0)
     else:
         assert _nw(A1, A0) > 1e-3
-        # viscosity dissipates: its work on the flow is negative
+        # viscosity dissipates: its work on the flow is negative, and it heats every
+        # particle (each pair approaches, so each pair's heating term is >= 0)
         v = np.stack([st[0]["vx"], st[0]["vy"], st[0]["vz"]], 1)
         assert np.sum(st[0]["m"] * np.sum(v * (A1 - A0), 1)) < 0
+        e0 = _gas_rates(oracle_mod, params, st, alpha=0.0, beta=0.0)[1]
+        e1 = _gas_rates(oracle_mod, params, st, alpha=1.0, beta=2.0)[1]
+        heat = e1 - e0
+        assert np.all(heat >= -1e-12 * np.max(np.abs(heat)))
+        assert np.mean(heat > 0) > 0.9
 
 
 def _pair_state(d, h, u, e=(1.0, 1.0)):
