@@ -112,9 +112,6 @@ Engine::~Engine() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : allocs_) (void)hipFree(p);
-    for (hipEvent_t e : side_ev_)
-        if (e) (void)hipEventDestroy(e);
-    if (side_stream_) (void)hipStreamDestroy(side_stream_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -313,17 +310,13 @@ int Engine::prepare() {
     a_.tile = tile_;
     if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
     a_.pband = nullptr;
+    a_.bperm = nullptr;
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].precision == DSH_MIXED && tc_h_[t].wall_band) {
             if ((rc = alloc(&a_.pband, (size_t)ntot_))) return rc;
+            if ((rc = alloc(&a_.bperm, bperm_ints(a_.pmap64.nblocks)))) return rc;
             break;
         }
-    if (a_.pband) {
-        if ((rc = hip_check(hipStreamCreateWithFlags(&side_stream_, hipStreamNonBlocking), "hipStreamCreate")))
-            return rc;
-        for (hipEvent_t& e : side_ev_)
-            if ((rc = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"))) return rc;
-    }
     a_.ginv = nullptr;
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].drag_mode == 0) {
@@ -496,8 +489,6 @@ int Engine::enqueue_step(bool timed) {
         (void)hipEventRecord(ev[0], stream_);
         ++pending_timed_;
     }
-    const SidePass side_d{side_stream_, side_ev_[0], side_ev_[1]};
-    const SidePass side_f{side_stream_, side_ev_[2], side_ev_[3]};
     for (int i = 0; i < kStages; ++i) {
         hipError_t e = hipSuccess;
         switch (i) {
@@ -506,8 +497,8 @@ int Engine::enqueue_step(bool timed) {
             case 2: e = launch_scatter(a_, stream_); break;
             case 3: e = launch_rank_permute(a_, stream_); break;
             case 4: e = launch_radius(a_, stream_); break;
-            case 5: e = launch_density(a_, precision_, stats_, stream_, side_stream_ ? &side_d : nullptr); break;
-            case 6: e = launch_force(a_, precision_, stream_, side_stream_ ? &side_f : nullptr); break;
+            case 5: e = launch_density(a_, precision_, stats_, stream_); break;
+            case 6: e = launch_force(a_, precision_, stream_); break;
             case 7: e = launch_idic(a_, stream_); break;
             case 8: e = launch_finalize(a_, stream_); break;
         }

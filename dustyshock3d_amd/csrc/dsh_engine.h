@@ -108,9 +108,6 @@ private:
     std::vector<void*> allocs_;
     StepArgs a_{};
     hipStream_t stream_ = nullptr;
-    // FP64 wall-band pair kernels run beside the FP32 ones (mixed precision with a band)
-    hipStream_t side_stream_ = nullptr;
-    hipEvent_t side_ev_[4] = {nullptr, nullptr, nullptr, nullptr};   // density fork/join, force fork/join
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;

@@ -62,7 +62,12 @@ struct StepArgs {
     // mixed precision with an FP64 wall band: per sorted particle, 1 if its 64-particle chunk
     // has a particle in its tube's band (written by k_band each step); null otherwise
     unsigned char* pband;
+    // with pband: the chunk order of the fused pair kernels (pmap64.nblocks entries, band count,
+    // gas band count), then per-chunk band flags (pmap64.nblocks) and per-block counts
+    int* bperm;
 };
+// ints of StepArgs::bperm for nall pmap64 chunks
+inline size_t bperm_ints(int nall) { return 2 * (size_t)nall + 2 + (size_t)(nall + 1023) / 1024 + 1; }
 
 // scratch: kPrepParts doubles per particle block of pmap
 constexpr int kPrepParts = 7;
@@ -72,15 +77,8 @@ hipError_t launch_scan(const StepArgs& a, hipStream_t s);
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s);
 hipError_t launch_rank_permute(const StepArgs& a, hipStream_t s);
 hipError_t launch_radius(const StepArgs& a, hipStream_t s);
-// A second stream for the FP64 wall-band instance of the tiled pair kernels (mixed precision):
-// it forks from the step stream after the event `fork` and the step stream waits for `join`, so
-// the band's few FP64 waves run beside the FP32 waves instead of after them.
-struct SidePass {
-    hipStream_t stream;
-    hipEvent_t fork, join;
-};
-hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s, const SidePass* side);
-hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s, const SidePass* side);
+hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s);
+hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s);
 hipError_t launch_idic(const StepArgs& a, hipStream_t s);
 int idic_run_cells();     // averaging cells per wave of the f == 1 drag kernel
 int idic_block_cells();   // averaging cells per block of it (one conservation partial each)

@@ -750,7 +750,8 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
 // step.  Band chunks run the FP64 pair kernels; FP32 chunks evaluate their pairs with band
 // (and frozen) particles in FP64, so both particles of a pair compute its terms alike.
 __global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict__ td, const int* __restrict__ cur,
-                                              PBuf B, unsigned char* __restrict__ pband) {
+                                              PBuf B, unsigned char* __restrict__ pband, const int* __restrict__ blk64,
+                                              int* __restrict__ cflag) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     const bool valid = li < ip.sg.n;
@@ -763,6 +764,70 @@ __global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict_
     }
     const bool any = __ballot(in) != 0ull;
     if (valid) pband[ip.sg.off + li] = any ? 1 : 0;
+    // the chunk's flag by its pmap64 block (pmap and pmap64 list the same segments, and both
+    // start each segment on a chunk boundary)
+    if (cflag != nullptr && (threadIdx.x & 63) == 0) {
+        const int ci = li >> 6;
+        if (ci < ((ip.sg.n + 63) >> 6)) cflag[blk64[find_seg(mp, blockIdx.x)] + ci] = any ? 1 : 0;
+    }
+}
+
+// Chunk order of the fused mixed-precision pair kernel: the band's chunks ascending, then the
+// others ascending, in perm[0, nall), with the band count at perm[nall] and the count of band
+// chunks below gas_blocks (the gas chunks) at perm[nall + 1].  Two passes over blocks of
+// kPermThreads chunks: counts per block, then each block writes its chunks at its offsets.
+constexpr int kPermThreads = 1024;
+__device__ __forceinline__ int block_sum_1024(int v, int* sh) {
+    const int w = threadIdx.x >> 6;
+    int t = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[w] = t;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int k = 0; k < kPermThreads / 64; ++k) r += sh[k];
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(kPermThreads) void k_perm_count(const int* __restrict__ cflag, int nall,
+                                                            int* __restrict__ bcnt) {
+    __shared__ int sh[kPermThreads / 64];
+    const int b = blockIdx.x * kPermThreads + threadIdx.x;
+    const int c = block_sum_1024(b < nall ? cflag[b] : 0, sh);
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
+}
+__global__ __launch_bounds__(kPermThreads) void k_perm_write(const int* __restrict__ cflag, int nall,
+                                                            const int* __restrict__ bcnt, int gas_blocks,
+                                                            int* __restrict__ perm) {
+    __shared__ int sh[kPermThreads / 64];
+    __shared__ int wpre[kPermThreads / 64];
+    // band chunks of the blocks before this one, and of all blocks (nbk <= a few hundred)
+    int before = 0, total = 0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += kPermThreads) {
+        const int v = bcnt[k];
+        total += v;
+        before += k < (int)blockIdx.x ? v : 0;
+    }
+    before = block_sum_1024(before, sh);
+    total = block_sum_1024(total, sh);
+    const int b = blockIdx.x * kPermThreads + threadIdx.x;
+    const int f = b < nall ? cflag[b] : 0;
+    // band chunks before b inside the block: wave prefix by ballot, then the waves before
+    const unsigned long long m = __ballot(f != 0);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wpre[w] = __popcll(m);
+    __syncthreads();
+    int waves_before = 0;
+    for (int k = 0; k < w; ++k) waves_before += wpre[k];
+    const int band_before = before + waves_before + in_wave;
+    if (b < nall) {
+        if (f) perm[band_before] = b;
+        else perm[total + (b - band_before)] = b;
+        if (b == nall - 1) perm[nall] = total;
+        if (b == gas_blocks - 1) perm[nall + 1] = band_before + f;
+    }
 }
 
 template <typename R, bool kStats>
@@ -1046,7 +1111,8 @@ struct TileLds {
 // not match the instance's precision is left to the other instance; the force pair of two
 // band particles is FP64, of a frozen and an active particle FP64, of any other two FP32.
 template <typename R, int KIND, bool kStats, int kTileP>
-__device__ __forceinline__ void pairs_chunk(int b, const Map& mp, const TubeConst* __restrict__ tc,
+__device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L, int b, const Map& mp,
+                                            const TubeConst* __restrict__ tc,
                                             const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                             const PBuf& B, const int* __restrict__ start,
                                             const int* __restrict__ wkey, const double* __restrict__ reach,
@@ -1056,7 +1122,6 @@ __device__ __forceinline__ void pairs_chunk(int b, const Map& mp, const TubeCons
                                             const unsigned char* __restrict__ pband) {
     static_assert(kTileP <= 256, "window lengths are reduced with wave_max_u9");
     static_assert(kTileP >= 64, "a density window of 64 slots fits in one piece");
-    __shared__ TileLds<R, KIND, kStats, kTileP> L;
     const int sgi = find_seg(mp, b);
     const Seg sg = mp.seg[sgi];
     const int lane = threadIdx.x;
@@ -1658,8 +1723,71 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
     unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
     const unsigned char* __restrict__ pband) {
-    pairs_chunk<R, KIND, kStats, kTileP>(xcd_remap(blockIdx.x, gridDim.x), mp, tc, td, cur, B, start, wkey, reach, pr2,
-                                         csav, ncount, nhash, masks, mask_chunks, pband);
+    __shared__ TileLds<R, KIND, kStats, kTileP> L;
+    pairs_chunk<R, KIND, kStats, kTileP>(L, xcd_remap(blockIdx.x, gridDim.x), mp, tc, td, cur, B, start, wkey, reach,
+                                         pr2, csav, ncount, nhash, masks, mask_chunks, pband);
+}
+
+// Block b of the part [p0, p1) of a grid, remapped so that each XCD (which takes the blocks
+// b with b % 8 equal to its index) runs one contiguous stretch of the part.
+__device__ __forceinline__ int part_remap(int b, int p0, int p1) {
+    const int x = b & 7;
+    int before = 0;
+#pragma unroll
+    for (int y = 0; y < 8; ++y) {
+        const int first = p0 + ((y - p0) & 7);
+        const int cnt = first < p1 ? (p1 - first + 7) >> 3 : 0;
+        before += y < x ? cnt : 0;
+    }
+    const int first = p0 + ((x - p0) & 7);
+    return p0 + before + ((b - first) >> 3);
+}
+
+// Pieces of the FP64 wall-band path inside the fused kernels: the largest of 128, 96, 64
+// candidates whose FP64 force tile fits in the FP32 force tile's LDS (its records are about
+// twice the FP32 ones).  The density pass uses the same size: the force pass replays the
+// density pass's stored hit masks, whose slots follow the piece layout.
+template <int kTileP>
+constexpr int band_tile() {
+    constexpr size_t f32 = sizeof(TileLds<float, 1, false, kTileP>);
+    if (sizeof(TileLds<double, 1, false, 128>) <= f32) return 128;
+    if (sizeof(TileLds<double, 1, false, 96>) <= f32) return 96;
+    return 64;
+}
+
+// Mixed precision with an FP64 wall band, both paths in one launch: perm (built by
+// k_perm_write this step) lists the band's chunks first, then the others, each part in
+// ascending chunk order.  Blocks below the band count run the FP64 path on band chunks and
+// are dispatched first, so the band's slower waves overlap the FP32 ones instead of trailing
+// them; the rest run the FP32 path.  For the gas force map (KIND 1) the band part is the gas
+// band chunks (perm[nall + 1] of them) and the FP32 part starts at perm[nband_all].  The two
+// paths share one LDS allocation.
+template <int KIND, bool kStats, int kTileP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_fused(
+    Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+    const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
+    double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
+    unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
+    const unsigned char* __restrict__ pband, const int* __restrict__ perm, int nall) {
+    constexpr int kBandTile = band_tile<kTileP>();
+    using L32 = TileLds<float, KIND, kStats, kTileP>;
+    using L64 = TileLds<double, KIND, kStats, kBandTile>;
+    __shared__ __align__(16) unsigned char raw[sizeof(L32) > sizeof(L64) ? sizeof(L32) : sizeof(L64)];
+    const int nb = gridDim.x;
+    const int band_all = uni(perm[nall]);
+    const int nband = KIND == 0 ? band_all : uni(perm[nall + 1]);
+    const int b = blockIdx.x;
+    if (b < nband) {
+        const int e = part_remap(b, 0, nband);
+        pairs_chunk<double, KIND, kStats, kBandTile>(*reinterpret_cast<L64*>(raw), uni(perm[e]), mp, tc, td, cur, B,
+                                                     start, wkey, reach, pr2, csav, ncount, nhash, masks, mask_chunks,
+                                                     pband);
+    } else {
+        const int e = part_remap(b, nband, nb);
+        pairs_chunk<float, KIND, kStats, kTileP>(*reinterpret_cast<L32*>(raw), uni(perm[band_all + (e - nband)]), mp,
+                                                 tc, td, cur, B, start, wkey, reach, pr2, csav, ncount, nhash, masks,
+                                                 mask_chunks, pband);
+    }
 }
 
 // kernel.cu:2310-2437 for one component; n_dust 1 or 2, reference operation order.
@@ -2228,9 +2356,21 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, 0);
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp, a.wtmp + a.ncells, 1);
     hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp + a.ncells, a.reach);
-    // chunk flags of the FP64 wall band (mixed precision), read by both pair passes
-    if (a.pband)
-        hipLaunchKernelGGL(k_band, dim3(a.pmap.nblocks), b, 0, s, a.pmap, a.td, a.cur, a.B, a.pband);
+    // chunk flags of the FP64 wall band (mixed precision) and the chunk order of the fused
+    // pair kernels (band chunks first), read by both pair passes
+    if (a.pband) {
+        const int nall = a.pmap64.nblocks;
+        hipLaunchKernelGGL(k_band, dim3(a.pmap.nblocks), b, 0, s, a.pmap, a.td, a.cur, a.B, a.pband, a.pmap64.blk,
+                           a.bperm ? a.bperm + nall + 2 : nullptr);
+        if (a.bperm && nall > 0) {
+            const int nbk = (nall + kPermThreads - 1) / kPermThreads;
+            int* cflag = a.bperm + nall + 2;
+            int* bcnt = cflag + nall;
+            hipLaunchKernelGGL(k_perm_count, dim3(nbk), dim3(kPermThreads), 0, s, cflag, nall, bcnt);
+            hipLaunchKernelGGL(k_perm_write, dim3(nbk), dim3(kPermThreads), 0, s, cflag, nall, bcnt,
+                               a.gmap64.nblocks, a.bperm);
+        }
+    }
     return hipGetLastError();
 }
 
@@ -2239,6 +2379,24 @@ static void launch_tiled_t(const StepArgs& a, const Map& m, const unsigned char*
     hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
                        a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
                        pband);
+}
+
+template <int KIND, bool kStats, int kTileP>
+static void launch_fused_t(const StepArgs& a, const Map& m, hipStream_t s) {
+    hipLaunchKernelGGL((k_pairs_fused<KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
+                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
+                       a.pband, a.bperm, a.pmap64.nblocks);
+}
+
+template <int KIND, bool kStats>
+static void launch_fused(const StepArgs& a, const Map& m, hipStream_t s) {
+    switch (a.tile) {
+        case 128: launch_fused_t<KIND, kStats, 128>(a, m, s); break;
+        case 192: launch_fused_t<KIND, kStats, 192>(a, m, s); break;
+        case 224: launch_fused_t<KIND, kStats, 224>(a, m, s); break;
+        case 256: launch_fused_t<KIND, kStats, 256>(a, m, s); break;
+        default: launch_fused_t<KIND, kStats, 160>(a, m, s); break;
+    }
 }
 
 template <typename R, int KIND, bool kStats>
@@ -2252,39 +2410,25 @@ static void launch_tiled1(const StepArgs& a, const Map& m, const unsigned char* 
     }
 }
 
-// FP64 mode: the FP64 instance over every chunk.  Mixed: the FP32 instance over the chunks
-// outside the wall band and, with a band, the FP64 instance over the band's chunks; the two
-// write disjoint chunks, so the FP64 one runs on the side stream beside the FP32 one.
+// FP64 mode: the FP64 instance over every chunk.  Mixed without a wall band: the FP32
+// instance.  Mixed with a band: the fused kernel, band chunks on its FP64 path first.
 template <int KIND, bool kStats>
-static hipError_t launch_tiled(const StepArgs& a, const Map& m, int precision, hipStream_t s, const SidePass* side) {
+static hipError_t launch_tiled(const StepArgs& a, const Map& m, int precision, hipStream_t s) {
     if (precision == 0) {
         launch_tiled1<double, KIND, kStats>(a, m, nullptr, s);
-        return hipGetLastError();
-    }
-    if (!a.pband) {
+    } else if (!a.pband) {
         launch_tiled1<float, KIND, kStats>(a, m, nullptr, s);
-        return hipGetLastError();
+    } else {
+        launch_fused<KIND, kStats>(a, m, s);
     }
-    hipStream_t s64 = side ? side->stream : s;
-    hipError_t e;
-    if (side) {
-        if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
-    }
-    launch_tiled1<double, KIND, kStats>(a, m, a.pband, s64);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (side && (e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
-    launch_tiled1<float, KIND, kStats>(a, m, a.pband, s);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (side && (e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
-    return hipSuccess;
+    return hipGetLastError();
 }
 
-hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s, const SidePass* side) {
+hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStream_t s) {
     if (a.variant == kPairTiled) {
         if (a.pmap64.nblocks == 0) return hipSuccess;
-        return stats ? launch_tiled<0, true>(a, a.pmap64, precision, s, side)
-                     : launch_tiled<0, false>(a, a.pmap64, precision, s, side);
+        return stats ? launch_tiled<0, true>(a, a.pmap64, precision, s)
+                     : launch_tiled<0, false>(a, a.pmap64, precision, s);
     }
     const dim3 g(a.pmap.nblocks), b(kBS);
     if (precision == 0) {
@@ -2305,10 +2449,10 @@ hipError_t launch_density(const StepArgs& a, int precision, bool stats, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s, const SidePass* side) {
+hipError_t launch_force(const StepArgs& a, int precision, hipStream_t s) {
     if (a.variant == kPairTiled) {
         if (a.gmap64.nblocks == 0) return hipSuccess;
-        return launch_tiled<1, false>(a, a.gmap64, precision, s, side);
+        return launch_tiled<1, false>(a, a.gmap64, precision, s);
     }
     if (a.gmap.nblocks == 0) return hipSuccess;
     const dim3 g(a.gmap.nblocks), b(kBS);

@@ -264,6 +264,7 @@ int step_noalloc(const dsh_params* P, const dsh_soa_view* in, const int* counts,
     a.masks = A.take<unsigned long long>(mask_words(R.maps[6].blk[1]));
     a.ginv = P->drag_mode == 0 ? A.take<int>((size_t)counts[0]) : nullptr;
     a.pband = (P->precision == DSH_MIXED && C.wall_band) ? A.take<unsigned char>((size_t)ntot) : nullptr;
+    a.bperm = a.pband ? A.take<int>(bperm_ints(R.maps[5].blk[R.maps[5].nseg])) : nullptr;
     double* prep = A.take<double>((size_t)kPrepParts * R.maps[0].blk[nsp]);
     Map* dmaps[] = {&a.pmap, &a.gmap, &a.cmap, &a.amap, &a.amap_runs, &a.pmap64, &a.gmap64};
     for (int i = 0; i < 7; ++i) {
@@ -309,8 +310,8 @@ int step_noalloc(const dsh_params* P, const dsh_soa_view* in, const int* counts,
         hipError_t e;
         if ((e = launch_key(a, st)) != hipSuccess || (e = launch_scan(a, st)) != hipSuccess ||
             (e = launch_scatter(a, st)) != hipSuccess || (e = launch_rank_permute(a, st)) != hipSuccess ||
-            (e = launch_radius(a, st)) != hipSuccess || (e = launch_density(a, P->precision, false, st, nullptr)) != hipSuccess ||
-            (e = launch_force(a, P->precision, st, nullptr)) != hipSuccess || (e = launch_idic(a, st)) != hipSuccess ||
+            (e = launch_radius(a, st)) != hipSuccess || (e = launch_density(a, P->precision, false, st)) != hipSuccess ||
+            (e = launch_force(a, P->precision, st)) != hipSuccess || (e = launch_idic(a, st)) != hipSuccess ||
             (e = launch_finalize(a, st)) != hipSuccess)
             return DSH_ERR_HIP;
     }
