@@ -63,11 +63,12 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
 def pass_kernels(stage, tile=DEFAULT_TILE):
     """Kernel names (as rocprofv3 reports them) of one pass of the step."""
     return {
-        "density": [f"k_pairs_tiled<float, 0, false, {tile}>"],
-        "force": [f"k_pairs_tiled<float, 1, false, {tile}>"],
+        # mixed precision with the FP64 wall band: one fused launch per pair pass
+        "density": [f"k_pairs_fused<0, false, {tile}>"],
+        "force": [f"k_pairs_fused<1, false, {tile}>"],
         "idic": ["k_idic_runs<1>"],
         "sort": ["k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
-                 "k_window_max", "k_radius"],
+                 "k_window_max", "k_radius", "k_band", "k_perm_count", "k_perm_write"],
     }.get(stage, [])
 
 
