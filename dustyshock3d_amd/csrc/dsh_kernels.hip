@@ -98,12 +98,33 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // ---------------------------------------------------------------- real-type helpers
+// The pair kernels take reciprocals and square roots of positive, normal FP64 values only
+// (smoothing lengths, densities, squared distances of distinct particles; a zero distance is
+// handled by the caller).  For those the hardware approximations refined by Newton steps are
+// within an ulp of the correctly rounded result, without the scaling and special-case
+// fix-ups of the IEEE sequences (about a third of their instructions).
+// 1/x: v_rcp_f64, then two Newton steps y += y (1 - x y).
 template <typename R> __device__ __forceinline__ R rcp(R x);
-template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ double rcp<double>(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
+    y = __builtin_fma(y, __builtin_fma(-x, y, 1.0), y);
+    return y;
+}
 template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// sqrt(x): Goldschmidt iteration from v_rsq_f64 (g -> sqrt(x), h -> 1/(2 sqrt(x))), then one
+// residual correction g += (x - g^2) h; 0 for x = 0 (the density's self pair).
 template <typename R> __device__ __forceinline__ R sqrt_r(R x);
-template <> __device__ __forceinline__ double sqrt_r<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ double sqrt_r<double>(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(__builtin_fma(-g, g, x), h, g);
+    return x > 0.0 ? g : 0.0;
+}
 // v_sqrt_f32 (1 ulp); arguments are squared distances, never denormal-small in practice.
 template <> __device__ __forceinline__ float sqrt_r<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -719,7 +740,7 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
                                             bool plain, T& coef, T& edc) {
     T dist, rinv, dist2;
     if constexpr (sizeof(T) == 8) {
-        dist = sqrt(d2);
+        dist = sqrt_r<T>(d2);
         rinv = rcp<T>(dist);
         dist2 = dist * dist;
     } else {
