@@ -111,6 +111,8 @@ Engine::~Engine() {
     for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    if (graph_exec_n_) (void)hipGraphExecDestroy(graph_exec_n_);
+    if (graph_n_) (void)hipGraphDestroy(graph_n_);
     for (void* p : allocs_) (void)hipFree(p);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -528,28 +530,30 @@ int Engine::collect_times() {
     return DSH_OK;
 }
 
-int Engine::build_graph() {
-    if (graph_exec_) {
-        (void)hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
+// Captures `steps` consecutive steps of the step stream into one executable graph.
+int Engine::build_graph(int steps, hipGraph_t& graph, hipGraphExec_t& exec) {
+    if (exec) {
+        (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
     }
-    if (graph_) {
-        (void)hipGraphDestroy(graph_);
-        graph_ = nullptr;
+    if (graph) {
+        (void)hipGraphDestroy(graph);
+        graph = nullptr;
     }
     int rc;
     if ((rc = hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "begin capture")))
         return rc;
-    const int r = enqueue_step(false);
+    int r = 0;
+    for (int k = 0; k < steps && r == 0; ++k) r = enqueue_step(false);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(stream_, &g);
-    if (r) return r;
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+    }
     if ((rc = hip_check(e, "end capture"))) return rc;
-    graph_ = g;
-    if ((rc = hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "graph instantiate")))
-        return rc;
-    graph_dirty_ = false;
-    return DSH_OK;
+    graph = g;
+    return hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
 }
 
 int Engine::launch_steps(int nsteps) {
@@ -559,9 +563,16 @@ int Engine::launch_steps(int nsteps) {
         if (timing_) {
             if ((rc = enqueue_step(true))) return rc;
         } else if (use_graph_) {
-            if (graph_dirty_ || !graph_exec_)
-                if ((rc = build_graph())) return rc;
-            if ((rc = hip_check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch"))) return rc;
+            if (graph_dirty_) {
+                if ((rc = build_graph(1, graph_, graph_exec_))) return rc;
+                if ((rc = build_graph(kGraphSteps, graph_n_, graph_exec_n_))) return rc;
+                graph_dirty_ = false;
+            }
+            // kGraphSteps steps per launch while that many are left
+            const bool multi = nsteps - k >= kGraphSteps;
+            if ((rc = hip_check(hipGraphLaunch(multi ? graph_exec_n_ : graph_exec_, stream_), "hipGraphLaunch")))
+                return rc;
+            if (multi) k += kGraphSteps - 1;
         } else {
             if ((rc = enqueue_step(false))) return rc;
         }

@@ -93,7 +93,7 @@ private:
     int hip_check(hipError_t e, const char* what);
     template <class T> int alloc(T** p, size_t count);
     int enqueue_step(bool timed);
-    int build_graph();
+    int build_graph(int steps, hipGraph_t& graph, hipGraphExec_t& exec);
 
     std::vector<dsh_params> params_;
     std::vector<HostSpecies> staged_;   // [tube * kMaxSpecies + species]
@@ -108,8 +108,10 @@ private:
     std::vector<void*> allocs_;
     StepArgs a_{};
     hipStream_t stream_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
+    // one step, and kGraphSteps steps, captured as graphs (rebuilt when graph_dirty_)
+    static constexpr int kGraphSteps = 8;
+    hipGraph_t graph_ = nullptr, graph_n_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr, graph_exec_n_ = nullptr;
     bool prepared_ = false, stats_ = false, timing_ = false, use_graph_ = true, graph_dirty_ = true;
     int precision_ = 0;
     int pair_variant_ = kPairTiled;

@@ -157,3 +157,24 @@ def test_mixed_physics_batch_matches_alone():
             for k in FIELDS:
                 assert np.array_equal(arrs[k], ref[k]), (t, s, k)
     lib.dsh_batch_destroy(h)
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_graph_replays_match_eager_launches(precision):
+    """Steps replayed from the captured graphs (eight steps per launch while eight are left,
+    then single steps) equal steps launched one kernel at a time, bit for bit, with eager
+    work and synchronisations between the replays."""
+    spec = configs.cfg0(precision=precision, max_steps=40)
+    g = Simulation.from_spec(spec)
+    e = Simulation.from_spec(spec)
+    e.set_timing(True)          # stage events: every kernel launched eagerly
+    for n in (19, 1, 8, 5):     # 2 x 8 + 3, 1, 8, 5: both graphs, several replays each
+        print(f"\ngraph vs eager precision={precision} steps={n}", flush=True)
+        g.step(n)
+        e.step(n)
+        assert g.status() == e.status()
+        for s in (0, 1):
+            dg, de = g.download(s), e.download(s)
+            for k in FIELDS:
+                assert np.array_equal(dg[k], de[k]), (n, s, k)
+        assert np.array_equal(g.conservation(), e.conservation())
