@@ -98,9 +98,13 @@ typedef struct dsh_params {
        the frozen layers (f = avg_factor) run their density and force in FP64 (per 64-particle
        chunk; a pair is FP64
        when both particles are in the band or one is frozen), so the wall force, and with it
-       the total momentum, follows an FP64 run: to 1e-13 per step, and over a run until the
-       first particle is assigned to another averaging cell than in the FP64 run.  The band's
-       time term needs a finite run (t_end < 1e29 or max_steps > 0); without one, 3 + f cells. */
+       the total momentum, follows an FP64 run: to 1e-13 per step.  Over a run it does until
+       FP32 rounding from outside the band reaches the wall layers (a cfg0-sized 50-step tube:
+       <= 1e-9 for 25 steps at speed 0.5, 4e-11 after 50 steps at speed 2) and until a particle
+       is assigned to another averaging cell than in the FP64 run; from then on, cells that take
+       the reference's fallback drag (kernel.cu:2160-2211) move the total momentum by
+       particle-sized amounts.  The band's time term needs a finite run (t_end < 1e29 or
+       max_steps > 0); without one, 3 + f cells. */
     int    fp64_wall_band;        /* 1 (default) or 0 */
     double wall_band_speed;       /* band edge speed in units of cs (default 0.5) */
 } dsh_params;
