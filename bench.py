@@ -339,8 +339,10 @@ class SweepArgs:
 
 
 class ServeArgs:
-    """Defaults of the cfg4 stream as the main benchmark line runs it."""
-    variant, requests, load, sat_rate, budget, epoch, seed = 1, 60, 0.8, 0.0, 200e9, 8, 7
+    """Defaults of the cfg4 stream as the main benchmark line runs it.  heavy: requests above
+    this many estimated particle-steps start one at a time (the large tubes, ~3.9e8; the sweep
+    tubes stay below 4e7)."""
+    variant, requests, load, sat_rate, budget, epoch, seed, heavy = 1, 60, 0.8, 0.0, 200e9, 8, 7, 1e8
 
 
 def sweep_result(args) -> dict:
@@ -431,7 +433,8 @@ def serve_result(args) -> dict:
     # continuous batching
     t0 = time.perf_counter()
     clock = lambda: time.perf_counter() - t0  # noqa: E731
-    sv = Server(hbm_budget=args.budget, steps_per_epoch=args.epoch, pair_variant=args.variant, clock=clock)
+    sv = Server(hbm_budget=args.budget, steps_per_epoch=args.epoch, pair_variant=args.variant, clock=clock,
+                heavy_work=args.heavy)
     ids, k = [], 0
     while True:
         now = clock()
@@ -464,7 +467,10 @@ def serve_result(args) -> dict:
         "rebuild_s": st["rebuild_s"], "advance_s": st["advance_s"], "hbm_budget": args.budget,
         "dtype": "f32 pair terms accumulated in f64 + f64 wall band, drag, positions, reductions", "data": "synthetic",
         "config": {"workload": "cfg4: Poisson arrivals, 95% cfg3-distribution tubes, 5% large tubes "
-                               "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch},
+                               "(y,z in [0,0.1], dx 0.002), each to t=0.2", "steps_per_epoch": args.epoch,
+                   "heavy_work": args.heavy,
+                   "policy": "continuous batching in arrival order under the HBM budget; requests above "
+                             "heavy_work particle-steps start one at a time"},
     }
 
 
@@ -507,6 +513,7 @@ if __name__ == "__main__":
         ap.add_argument("--budget", type=float, default=200e9)
         ap.add_argument("--epoch", type=int, default=8)
         ap.add_argument("--seed", type=int, default=7)
+        ap.add_argument("--heavy", type=float, default=1e8)
         a = ap.parse_args()
         import numpy as np  # noqa: F811
         {"sweep": sweep_mode, "serve": serve_mode, "copy": copy_mode}[a.mode](a)

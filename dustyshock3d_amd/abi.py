@@ -102,6 +102,7 @@ class ServerOpts(C.Structure):
         ("hbm_budget", C.c_double), ("hbm_capacity", C.c_double),
         ("steps_per_epoch", C.c_int), ("backend", C.c_int), ("pair_variant", C.c_int),
         ("clock", CLOCK_FN), ("clock_user", C.c_void_p), ("fake_fail_rebuild", C.c_int),
+        ("heavy_work", C.c_double), ("max_heavy", C.c_int),
     ]
 
 

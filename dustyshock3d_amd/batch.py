@@ -110,12 +110,13 @@ class Server:
 
     def __init__(self, hbm_budget: float, steps_per_epoch: int = 10, backend: int = abi.BACKEND_GPU,
                  clock: Optional[Callable[[], float]] = None, hbm_capacity: float = 0.0,
-                 pair_variant: int = abi.PAIR_TILED, fake_fail_rebuild: int = 0):
+                 pair_variant: int = abi.PAIR_TILED, fake_fail_rebuild: int = 0, heavy_work: float = 0.0,
+                 max_heavy: int = 1):
         self._lib = abi.lib()
         self._clock_cb = abi.CLOCK_FN(lambda _u: float(clock())) if clock else abi.CLOCK_FN()
         o = ServerOpts(hbm_budget=hbm_budget, hbm_capacity=hbm_capacity, steps_per_epoch=steps_per_epoch,
                        backend=backend, pair_variant=pair_variant, clock=self._clock_cb, clock_user=None,
-                       fake_fail_rebuild=fake_fail_rebuild)
+                       fake_fail_rebuild=fake_fail_rebuild, heavy_work=heavy_work, max_heavy=max_heavy)
         self._h = C.c_void_p()
         rc = self._lib.dsh_server_create(C.byref(o), C.byref(self._h))
         if rc != abi.OK:

@@ -2,8 +2,9 @@
 //
 // The running set is one engine batch.  A scheduling round (pump):
 //   1. admit queued requests in arrival order while the new batch and the old one
-//      (both live during the rebuild) fit the HBM budget; a request larger than the
-//      budget is rejected, otherwise the queue waits at the first that does not fit;
+//      (both live during the rebuild) fit the HBM budget and, for a heavy request, fewer
+//      than max_heavy heavy ones run; a request larger than the budget is rejected,
+//      otherwise the queue waits at the first that cannot start;
 //   2. if requests were admitted or cancelled, build a new batch: running tubes are
 //      adopted device to device (state, sorted order, clock), new tubes get their
 //      seeded set-up, finished tubes are dropped;
@@ -54,11 +55,22 @@ double tube_bytes(const dsh_tube_spec& spec) {
     return particles * per_particle + masks + cells + avg + 4096.0;
 }
 
+// Estimated steps of a request: max_steps, or t_end over the CFL step of the set-up at rest
+// (0.3 h / cs with h = 1.2 dx); the CFL step shrinks as the flow speeds up, so this is a lower
+// bound that orders requests by work.
+double est_steps(const dsh_tube_spec& spec) {
+    if (spec.max_steps > 0) return (double)spec.max_steps;
+    const double dt0 = spec.cfl * 1.2 * spec.dx_left;
+    return dt0 > 0 && spec.t_end < 1e29 ? std::ceil(spec.t_end / dt0) : 1e30;
+}
+
 struct Request {
     uint64_t id = 0;
     dsh_tube_spec spec{};
     int state = DSH_REQ_QUEUED;
     double arrival = 0, admitted = -1, finished = -1, particle_steps = 0, bytes = 0;
+    double work = 0;   // particles x estimated steps
+    bool heavy = false;
     int n[2] = {0, 0};
     std::vector<double> f[2][16];
     std::vector<int> ind[2];
@@ -164,8 +176,7 @@ public:
         for (int k : keep) s.push_back(slots_[k]);
         for (const Request* r : add) {
             Slot x;
-            const double dt0 = r->spec.cfl * 1.2 * r->spec.dx_left;
-            x.need = r->spec.max_steps > 0 ? r->spec.max_steps : (int)std::ceil(r->spec.t_end / dt0);
+            x.need = (int)est_steps(r->spec);
             x.bytes = tube_bytes(r->spec);
             s.push_back(x);
         }
@@ -212,6 +223,7 @@ public:
     Server(const dsh_server_opts& o, std::unique_ptr<Backend> be) : o_(o), be_(std::move(be)) {
         if (!o_.clock) o_.clock = wall_clock;
         if (o_.steps_per_epoch < 1) o_.steps_per_epoch = 1;
+        if (o_.max_heavy < 1) o_.max_heavy = 1;
     }
     double now() { return o_.clock(o_.clock_user); }
 
@@ -225,6 +237,8 @@ public:
         r.n[0] = n[0];
         r.n[1] = n[1];
         r.bytes = tube_bytes(spec);
+        r.work = (double)(n[0] + n[1]) * est_steps(spec);
+        r.heavy = o_.heavy_work > 0 && r.work > o_.heavy_work;
         if (r.bytes > o_.hbm_budget || r.bytes > o_.hbm_capacity) r.state = DSH_REQ_REJECTED;
         reqs_[r.id] = std::move(r);
         if (reqs_[next_id_ - 1].state == DSH_REQ_QUEUED) queue_.push_back(next_id_ - 1);
@@ -237,6 +251,7 @@ public:
         std::vector<int> keep;
         std::vector<uint64_t> keep_ids;
         double keep_bytes = 0;
+        int heavy = 0;
         bool changed = false;
         for (size_t s = 0; s < slot_ids_.size(); ++s) {
             Request& r = reqs_[slot_ids_[s]];
@@ -244,6 +259,7 @@ public:
                 keep.push_back((int)s);
                 keep_ids.push_back(r.id);
                 keep_bytes += r.bytes;
+                heavy += r.heavy ? 1 : 0;
             } else if (r.state == DSH_REQ_CANCELLED) {
                 changed = true;   // still stepped by the device: drop it now
             }
@@ -259,6 +275,8 @@ public:
                 continue;
             }
             if (new_bytes + r.bytes > o_.hbm_budget || new_bytes + r.bytes + old_bytes > o_.hbm_capacity) break;
+            if (r.heavy && heavy >= o_.max_heavy) break;
+            heavy += r.heavy ? 1 : 0;
             new_bytes += r.bytes;
             add.push_back(&r);
             add_ids.push_back(r.id);

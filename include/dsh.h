@@ -278,6 +278,14 @@ typedef struct dsh_server_opts {
     dsh_clock_fn clock;        /* NULL: monotonic wall clock in seconds */
     void* clock_user;
     int fake_fail_rebuild;     /* fake backend: the n-th rebuild (1-based) fails; 0 never */
+    /* Heavy requests: estimated work (particles x CFL steps to t_end, or max_steps) above
+       heavy_work.  At most max_heavy of them run at once (0 counts as 1), started in arrival
+       order; the queue waits behind a heavy request that cannot start, as behind one that does
+       not fit the budget.  Light requests keep batching.  Two heavy requests that share the GPU
+       both finish late; served one after the other, only the second waits.  heavy_work = 0:
+       no distinction. */
+    double heavy_work;
+    int max_heavy;
 } dsh_server_opts;
 int dsh_server_create(const dsh_server_opts* opts, dsh_server** out);
 int dsh_server_submit(dsh_server* sv, const dsh_tube_spec* spec, double arrival, uint64_t* id);

@@ -6,6 +6,7 @@ import pytest
 
 from dustyshock3d_amd import abi, configs
 from dustyshock3d_amd.batch import Server, footprint
+from dustyshock3d_amd.sim import tube_count
 
 
 class Clock:
@@ -132,3 +133,32 @@ def test_finished_tubes_cost_no_rebuild_of_their_own():
     assert sv.stats()["bytes_in_use"] == pytest.approx(2 * fp)
     run_until_idle(sv)
     assert sv.poll(a)["state"] == abi.REQ_DONE and sv.poll(c)["state"] == abi.REQ_DONE
+
+
+def test_heavy_requests_run_one_at_a_time_in_arrival_order():
+    """With heavy_work set, requests above it (particles x estimated steps) start one at a time
+    in arrival order; light requests behind a waiting heavy one wait too (FIFO), those ahead of
+    it batch with the running heavy one."""
+    heavy = [small(i, steps=300) for i in range(2)]
+    light = [small(10 + i, steps=20) for i in range(3)]
+    n = 2 * tube_count(heavy[0])
+    work = n * 100          # between the two classes: n x 300 and n x 20 particle-steps
+    sv = Server(hbm_budget=1e12, steps_per_epoch=10, backend=abi.BACKEND_FAKE, clock=Clock(), heavy_work=work)
+    h0 = sv.submit(heavy[0], 0.0)
+    l0 = sv.submit(light[0], 0.0)
+    h1 = sv.submit(heavy[1], 0.0)
+    l1 = sv.submit(light[1], 0.0)
+    r, q = sv.pump()
+    assert (r, q) == (2, 2)                      # h0 and l0 run; h1 waits, and l1 behind it
+    assert sv.poll(h1)["state"] == abi.REQ_QUEUED and sv.poll(l1)["state"] == abi.REQ_QUEUED
+    while sv.poll(h0)["state"] == abi.REQ_RUNNING:
+        assert sv.poll(h1)["state"] == abi.REQ_QUEUED
+        sv.pump()
+    run_until_idle(sv)
+    p = {k: sv.poll(k) for k in (h0, l0, h1, l1)}
+    assert all(v["state"] == abi.REQ_DONE for v in p.values())
+    assert p[h1]["admitted"] >= p[h0]["finished"]
+    # without the distinction both heavy requests start at once
+    sv2 = Server(hbm_budget=1e12, steps_per_epoch=10, backend=abi.BACKEND_FAKE, clock=Clock())
+    a, b = sv2.submit(heavy[0], 0.0), sv2.submit(heavy[1], 0.0)
+    assert sv2.pump() == (2, 0)
