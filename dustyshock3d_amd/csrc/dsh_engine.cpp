@@ -319,6 +319,15 @@ int Engine::prepare() {
             if ((rc = alloc(&a_.bperm, bperm_ints(a_.pmap64.nblocks)))) return rc;
             break;
         }
+    a_.frec = nullptr;
+    for (int t = 0; t < T; ++t)
+        if (tc_h_[t].precision == DSH_MIXED) {
+            // gas occupies the front of the sorted space (species-major)
+            long long ngas = 0;
+            for (const Seg& g : gseg_) ngas += g.n;
+            if ((rc = alloc(&a_.frec, (size_t)std::max(ngas, 1ll)))) return rc;
+            break;
+        }
     a_.ginv = nullptr;
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].drag_mode == 0) {

@@ -16,6 +16,18 @@ struct Map {
     int nblocks;
 };
 
+// Packed record of one gas particle in sorted order for the FP32 force pass, written by the
+// density pass of a mixed-precision tube: the force pass stages a candidate with four 16-byte
+// loads instead of gathering and converting eleven FP64 fields.
+struct alignas(16) ForceRec {
+    double x;
+    int y, z;          // fixed point: units of L / 2^30 from the box origin
+    float h, vx, vy, vz, m, rho, pr2, cs;
+    unsigned flags;    // bit 0 frozen, bit 1 in an FP64-band chunk, bit 2 y, z inside the fixed-point range
+    unsigned pad[3];
+};
+static_assert(sizeof(ForceRec) == 64, "four 16-byte loads");
+
 // pair-kernel variants: one thread per target with global-memory candidates, or one
 // wave per 64-target chunk with LDS-staged candidate rows
 constexpr int kPairDirect = 0;
@@ -56,6 +68,7 @@ struct StepArgs {
     unsigned idic_blocks;          // bit nd: a tube with averaging cells of f^3 > 1 neighbour cells, nd dust species
     unsigned idic_runs;            // bit nd: a tube with f == 1 and nd dust species
     unsigned long long* masks;   // gas density hit masks reused by the force pass
+    ForceRec* frec;        // per gas particle (sorted), mixed precision; null otherwise
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
     int variant;
     int tile;              // candidates staged per piece in the tiled pair kernels (kTiles)

@@ -1140,7 +1140,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                                             double* __restrict__ pr2, double* __restrict__ csav,
                                             int* __restrict__ ncount, unsigned long long* __restrict__ nhash,
                                             unsigned long long* __restrict__ masks, int mask_chunks,
-                                            const unsigned char* __restrict__ pband) {
+                                            const unsigned char* __restrict__ pband, ForceRec* __restrict__ frec) {
     static_assert(kTileP <= 256, "window lengths are reduced with wave_max_u9");
     static_assert(kTileP >= 64, "a density window of 64 slots fits in one piece");
     const int sgi = find_seg(mp, b);
@@ -1236,6 +1236,26 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
         bool out_of_range = false;
         for (int k = lane; k < n0 + n1; k += 64) {
             const int src = k < n0 ? s0 + k : s1 + (k - n0);
+            if constexpr (kFix) {
+                if (frec != nullptr) {
+                    const ForceRec r = frec[src];
+                    __builtin_amdgcn_sched_barrier(0);
+                    L.x[k] = r.x;
+                    L.y[k] = r.y;
+                    L.z[k] = r.z;
+                    L.h[k] = r.h;
+                    L.m[k] = r.m;
+                    L.vx[k] = r.vx;
+                    L.vy[k] = r.vy;
+                    L.vz[k] = r.vz;
+                    L.rho[k] = r.rho;
+                    L.pr2[k] = r.pr2;
+                    L.cs[k] = r.cs;
+                    L.ind[k] = (unsigned char)(r.flags & 3u);
+                    out_of_range = out_of_range || (r.flags & 4u) == 0u;
+                    continue;
+                }
+            }
             const long long g = wo + src;
             const double x = B.x[g], y = B.y[g], z = B.z[g], h = B.h[g], m = B.m[g];
             double vx = 0, vy = 0, vz = 0, rho = 0, p2 = 0, cs = 0;
@@ -1520,31 +1540,33 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     // ascend in x, a cell by x then id), so the candidates with -rch < x_j - x_i < rch (the
     // differences rounded as the predicate rounds them) are one run.  Any other candidate has
     // d2 >= dx^2 >= rch^2 > (h_i + h_j)^2 and cannot be a neighbour.  Returns the run as
-    // (start, length); binary searches on the staged x.
+    // (start, length); two binary searches on the staged x over [a, e), run side by side so
+    // that each step's two LDS reads are in flight together (the searches are chains of
+    // dependent reads; run one after the other they cost twice the round trips).
     auto xwindow = [&](int a, int e, int& len) {
         if (e <= a) {
             len = 0;
             return a;
         }
-        int lo = a, n = e - a;
-        while (n > 0) {
-            const int hh = n >> 1;
-            if (L.x[lo + hh] - xi <= -rch) {
-                lo += hh + 1;
-                n -= hh + 1;
-            } else {
-                n = hh;
+        int lo = a, nlo = e - a, hi = a, nhi = e - a;
+        while (nlo > 0 || nhi > 0) {
+            const int ml = lo + (nlo >> 1), mh = hi + (nhi >> 1);
+            const double vl = L.x[min(ml, e - 1)], vh = L.x[min(mh, e - 1)];
+            if (nlo > 0) {
+                if (vl - xi <= -rch) {
+                    lo = ml + 1;
+                    nlo -= (nlo >> 1) + 1;
+                } else {
+                    nlo >>= 1;
+                }
             }
-        }
-        int hi = lo;
-        n = e - lo;
-        while (n > 0) {
-            const int hh = n >> 1;
-            if (L.x[hi + hh] - xi < rch) {
-                hi += hh + 1;
-                n -= hh + 1;
-            } else {
-                n = hh;
+            if (nhi > 0) {
+                if (vh - xi < rch) {
+                    hi = mh + 1;
+                    nhi -= (nhi >> 1) + 1;
+                } else {
+                    nhi >>= 1;
+                }
             }
         }
         len = hi - lo;
@@ -1725,8 +1747,28 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
         }
         if (s == 0) {
             const double p = B.p[wo + q];
-            pr2[q] = p / (rho * rho);
-            csav[q] = sqrt(T.gamma * p / rho);
+            const double pr = p / (rho * rho), cs = sqrt(T.gamma * p / rho);
+            pr2[q] = pr;
+            csav[q] = cs;
+            if (frec != nullptr && T.precision == DSH_MIXED) {
+                // the FP32 force pass's staged values of this particle, converted once
+                ForceRec r;
+                r.x = xi;
+                r.y = fix_coord(yi, T.o[1], ups_y);
+                r.z = fix_coord(zi, T.o[2], ups_z);
+                r.h = (float)hi;
+                r.vx = (float)B.vx[wo + q];
+                r.vy = (float)B.vy[wo + q];
+                r.vz = (float)B.vz[wo + q];
+                r.m = (float)B.m[wo + q];
+                r.rho = (float)rho;
+                r.pr2 = (float)pr;
+                r.cs = (float)cs;
+                r.flags = (B.ind[wo + q] != 0 ? 1u : 0u) | (pband != nullptr && pband[q] != 0 ? 2u : 0u) |
+                          (fix_range_ok(yi, zi) ? 4u : 0u);
+                r.pad[0] = r.pad[1] = r.pad[2] = 0u;
+                frec[q] = r;
+            }
         }
     } else {
         B.ax[wo + q] = active ? a0.get() + d0 : 0.0;
@@ -1743,10 +1785,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
     double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
     unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
-    const unsigned char* __restrict__ pband) {
+    const unsigned char* __restrict__ pband, ForceRec* __restrict__ frec) {
     __shared__ TileLds<R, KIND, kStats, kTileP> L;
     pairs_chunk<R, KIND, kStats, kTileP>(L, xcd_remap(blockIdx.x, gridDim.x), mp, tc, td, cur, B, start, wkey, reach,
-                                         pr2, csav, ncount, nhash, masks, mask_chunks, pband);
+                                         pr2, csav, ncount, nhash, masks, mask_chunks, pband, frec);
 }
 
 // Block b of the part [p0, p1) of a grid, remapped so that each XCD (which takes the blocks
@@ -1789,7 +1831,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
     double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
     unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
-    const unsigned char* __restrict__ pband, const int* __restrict__ perm, int nall) {
+    const unsigned char* __restrict__ pband, const int* __restrict__ perm, int nall, ForceRec* __restrict__ frec) {
     constexpr int kBandTile = band_tile<kTileP>();
     using L32 = TileLds<float, KIND, kStats, kTileP>;
     using L64 = TileLds<double, KIND, kStats, kBandTile>;
@@ -1802,12 +1844,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
         const int e = part_remap(b, 0, nband);
         pairs_chunk<double, KIND, kStats, kBandTile>(*reinterpret_cast<L64*>(raw), uni(perm[e]), mp, tc, td, cur, B,
                                                      start, wkey, reach, pr2, csav, ncount, nhash, masks, mask_chunks,
-                                                     pband);
+                                                     pband, frec);
     } else {
         const int e = part_remap(b, nband, nb);
         pairs_chunk<float, KIND, kStats, kTileP>(*reinterpret_cast<L32*>(raw), uni(perm[band_all + (e - nband)]), mp,
                                                  tc, td, cur, B, start, wkey, reach, pr2, csav, ncount, nhash, masks,
-                                                 mask_chunks, pband);
+                                                 mask_chunks, pband, frec);
     }
 }
 
@@ -2399,14 +2441,14 @@ template <typename R, int KIND, bool kStats, int kTileP>
 static void launch_tiled_t(const StepArgs& a, const Map& m, const unsigned char* pband, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
                        a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
-                       pband);
+                       pband, a.frec);
 }
 
 template <int KIND, bool kStats, int kTileP>
 static void launch_fused_t(const StepArgs& a, const Map& m, hipStream_t s) {
     hipLaunchKernelGGL((k_pairs_fused<KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
                        a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
-                       a.pband, a.bperm, a.pmap64.nblocks);
+                       a.pband, a.bperm, a.pmap64.nblocks, a.frec);
 }
 
 template <int KIND, bool kStats>

@@ -174,6 +174,7 @@ int step_noalloc(const dsh_params* P, const dsh_soa_view* in, const int* counts,
         dry.take<int>(6 * (size_t)ntot);
         dry.take<double>(3 * (size_t)ntot);
         dry.take<unsigned long long>(mask_words((int)((ngas + 63) / 64)));
+        if (P->precision == DSH_MIXED) dry.take<ForceRec>(ngas);
         dry.take<double>((size_t)kPrepParts * (((size_t)ntot + 255) / 256 + nsp));
         if (!dry.ok) return DSH_ERR_BUDGET;
     }
@@ -262,6 +263,7 @@ int step_noalloc(const dsh_params* P, const dsh_soa_view* in, const int* counts,
     a.scan_bsum = A.take<int>((size_t)a.scan_nb);
     a.part = A.take<double>(5 * (size_t)C.navg);
     a.masks = A.take<unsigned long long>(mask_words(R.maps[6].blk[1]));
+    a.frec = P->precision == DSH_MIXED ? A.take<ForceRec>((size_t)counts[0]) : nullptr;
     a.ginv = P->drag_mode == 0 ? A.take<int>((size_t)counts[0]) : nullptr;
     a.pband = (P->precision == DSH_MIXED && C.wall_band) ? A.take<unsigned char>((size_t)ntot) : nullptr;
     a.bperm = a.pband ? A.take<int>(bperm_ints(R.maps[5].blk[R.maps[5].nseg])) : nullptr;
