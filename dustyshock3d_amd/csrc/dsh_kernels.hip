@@ -1179,6 +1179,15 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                   : nullptr;
     int segi = 0;
     int slot = 0;
+    // force pass: the stored x-windows of the next two segments and the hit masks of the next
+    // two slots, loaded together with a piece's candidates (so their memory round trip overlaps
+    // the staging instead of following it); pw_at / pm_at: the segment / slot they belong to
+    unsigned int pw[2] = {0u, 0u};
+    unsigned long long pm[2] = {0ull, 0ull};
+    int pw_at = -2, pm_at = -2;
+    // once the force pass searches a window itself (past the stored ones) it takes no stored
+    // mask either: masks follow the windows they were made for
+    bool win_own = false;
 
     Acc<R> a0, a1, a2, a3;  // density: a0; force: ax, ay, az, de/dt
     double d0 = 0.0, d1 = 0.0, d2s = 0.0, d3 = 0.0;  // force, FP64 terms of frozen neighbours
@@ -1233,6 +1242,18 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     // keeps the compiler from serialising them under register pressure).
     auto stage = [&](int s0, int n0, int s1, int n1) {
         __syncthreads();
+        if constexpr (KIND == 1) {
+            if (use_masks && segi < kMaskSlots) {
+                pw_at = segi;
+                pw[0] = wrow[(size_t)segi * 64];
+                pw[1] = segi + 1 < kMaskSlots ? wrow[(size_t)(segi + 1) * 64] : 0u;
+            }
+            if (use_masks && slot < kMaskSlots && !win_own) {
+                pm_at = slot;
+                pm[0] = mrow[(size_t)slot * 64];
+                pm[1] = slot + 1 < kMaskSlots ? mrow[(size_t)(slot + 1) * 64] : 0ull;
+            }
+        }
         bool out_of_range = false;
         for (int k = lane; k < n0 + n1; k += 64) {
             const int src = k < n0 ? s0 + k : s1 + (k - n0);
@@ -1307,8 +1328,8 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     auto window_mask = [&](int a, int len, int wn, const RowSeg& g, int gb) -> unsigned long long {
         unsigned long long mask = 0ull;
         if constexpr (KIND == 1) {
-            if (use_masks && slot < kMaskSlots) {
-                mask = mrow[(size_t)slot * 64];
+            if (use_masks && slot < kMaskSlots && !win_own) {
+                mask = slot == pm_at ? pm[0] : (slot == pm_at + 1 ? pm[1] : mrow[(size_t)slot * 64]);
             } else {
                 // no stored mask: each lane tests its own candidates only
                 const int nt = min(max(len, 0), 64);
@@ -1578,10 +1599,11 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     auto seg_window_range = [&](int a0, int e0, int& len) {
         int a;
         if (KIND == 1 && use_masks && segi < kMaskSlots) {
-            const unsigned int w = wrow[(size_t)segi * 64];
+            const unsigned int w = segi == pw_at ? pw[0] : (segi == pw_at + 1 ? pw[1] : wrow[(size_t)segi * 64]);
             a = (int)(w >> 16);
             len = (int)(w & 0xffffu);
         } else {
+            if (KIND == 1) win_own = true;
             a = xwindow(a0, e0, len);
             if (KIND == 0 && use_masks && segi < kMaskSlots) wrow[(size_t)segi * 64] = ((unsigned)a << 16) | (unsigned)len;
         }
