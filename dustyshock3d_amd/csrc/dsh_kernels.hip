@@ -749,20 +749,23 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
         dist2 = d2;
     }
     const T ih = rcp<T>(hh);
-    // (v_i - v_j).(x_i - x_j), IDIC.h:226-229
-    const T nu = dvx * (-fx) + dvy * (-fy) + dvz * (-fz);
+    // (v_i - v_j).(x_i - x_j), IDIC.h:226-229.  Explicit FMAs (not compiler contraction): the
+    // two orientations of a pair run the same operations on the same values (each product's
+    // operands both change sign), so the pair's terms stay mirror images bit for bit.
+    const T nu = fma_r(dvx, -fx, fma_r(dvy, -fy, dvz * (-fz)));
     T fnu = T(0);
     if (nu < T(0)) {
         const T cnu = T(0.5) * (csi + csj);
-        const T mu = hh * nu * rcp<T>(dist2 + epsv * hh * hh);
-        fnu = (-alpha * cnu * mu + beta * mu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
+        const T mu = hh * nu * rcp<T>(fma_r(epsv * hh, hh, dist2));
+        fnu = fma_r(beta * mu, mu, -alpha * cnu * mu) * rcp<T>(T(0.5) * (rhoi + rhoj));
     }
     const T dw = (T)kInvPi * (ih * ih * ih * ih) * spline_dw_hit<T>(dist * ih);
     const T dwr = dw * rinv;
+    const T ei = mi * fma_r(fnu, T(0.5), pr2i);
     if (plain)
-        edc = (mi * pr2i + mi * fnu * T(0.5)) * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
+        edc = ei * plain_edot_weight<T>(dvx, dvy, dvz, fx, fy, fz, ih);
     else
-        edc = (mi * pr2i + mi * fnu * T(0.5)) * nu * dwr;
+        edc = ei * nu * dwr;
     coef = mj * (pr2i + pr2j + fnu) * dwr;
 }
 
