@@ -56,7 +56,7 @@ class Params(C.Structure):
         ("dt_mode", C.c_int), ("cfl", C.c_double), ("max_steps", C.c_int),
         ("cell_size", C.c_double), ("cell_slack", C.c_double),
         ("avg_factor", C.c_int), ("precision", C.c_int),
-        ("fp64_wall_band", C.c_int), ("wall_band_speed", C.c_double),
+        ("fp64_wall_band", C.c_int), ("wall_band_speed", C.c_double), ("wall_band_horizon", C.c_int),
     ]
 
 

@@ -72,6 +72,7 @@ void tube_physics(const dsh_params& P, TubeConst& C) {
     C.precision = P.precision;
     C.wall_band = P.fp64_wall_band ? 1 : 0;
     C.band_speed = P.wall_band_speed > 0.0 ? P.wall_band_speed : 0.0;   // 0: the fixed part of the band alone
+    C.band_horizon = P.wall_band_horizon > 0 ? P.wall_band_horizon : 0;  // 0: the whole run
 }
 
 Engine::Engine(const std::vector<dsh_params>& params)

@@ -206,6 +206,7 @@ int dsh_default_params(dsh_params* p) {
     p->precision = DSH_FP64;
     p->fp64_wall_band = 1;
     p->wall_band_speed = 0.5;
+    p->wall_band_horizon = 50;
     return DSH_OK;
 }
 
@@ -378,7 +379,7 @@ int dsh_read_init(const char* path, dsh_params* p) {
                {'d', &p->eps_fallback[0]}, {'d', &p->eps_fallback[1]}, {'d', &p->tstop_fallback},
                {'i', &p->eps2_uses_count1}, {'i', &p->dt_mode},   {'d', &p->cfl},          {'i', &p->max_steps},
                {'d', &p->cell_size},      {'d', &p->cell_slack},  {'i', &p->avg_factor},   {'i', &p->precision},
-               {'i', &p->fp64_wall_band}, {'d', &p->wall_band_speed}};
+               {'i', &p->fp64_wall_band}, {'d', &p->wall_band_speed}, {'i', &p->wall_band_horizon}};
     for (auto& e : ext) {
         if (!std::fgets(s, sizeof s, f)) break;
         const int r = e.kind == 'i' ? std::sscanf(s, "%d", (int*)e.dst) : std::sscanf(s, "%lf", (double*)e.dst);

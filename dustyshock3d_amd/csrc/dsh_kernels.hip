@@ -318,20 +318,20 @@ __global__ __launch_bounds__(kBS) void k_prepare_part(Map mp, const TubeConst* _
 // through its frozen walls, so the particles next to them, within f * band_speed * cs * (time
 // left) + (3 + f) neighbour cells of the frozen layers (f: neighbour cells per averaging cell
 // along an axis), run in FP64: FP32 rounding made outside the band reaches the walls late or
-// not at all before the run ends.  An open-ended run keeps the fixed part; no band in FP64
-// mode or when switched off.
+// not at all before the run ends.  The time left counts at most band_horizon steps.  An
+// open-ended run keeps the fixed part; no band in FP64 mode or when switched off.
 __device__ __forceinline__ void set_band(const TubeConst& T, TubeDyn& D) {
     D.band_lo = -1e300;
     D.band_hi = 1e300;
     if (!T.wall_band || T.precision != DSH_MIXED) return;
     double left = 1e300;
     if (T.t_end < 1e29) left = T.t_end - D.t;
-    if (T.max_steps > 0) {
-        // a CFL step may grow: twice the current step per step left
-        const double by_steps = (double)(T.max_steps - D.steps) * D.dt * (T.dt_mode == DSH_DT_CFL ? 2.0 : 1.0);
-        left = fmin(left, by_steps);
-    }
+    // a CFL step may grow: twice the current step per step left
+    const double step_bound = D.dt * (T.dt_mode == DSH_DT_CFL ? 2.0 : 1.0);
+    if (T.max_steps > 0) left = fmin(left, (double)(T.max_steps - D.steps) * step_bound);
     if (!(left < 1e29)) left = 0.0;   // an open-ended run: the fixed part alone
+    // at most band_horizon steps ahead: later, averaging-cell membership differences decide
+    if (T.band_horizon > 0) left = fmin(left, (double)T.band_horizon * step_bound);
     // an averaging cell of f neighbour cells passes a particle's influence f cells on in one
     // step, so both terms scale with f
     const double w = T.band_speed * (double)T.f * T.cs * fmax(left, 0.0) + (double)(3 + T.f) * T.c[0];

@@ -47,6 +47,7 @@ struct TubeConst {
     int eos, h_mode, eps2_quirk, dt_mode, max_steps, precision;
     int wall_band;         // mixed precision: FP64 band next to the frozen walls (TubeDyn::band_lo/hi)
     double band_speed;     // band edge speed, units of cs
+    int band_horizon;      // steps the band's time term covers at most (0: the whole run)
 };
 
 struct TubeDyn {

@@ -104,9 +104,12 @@ typedef struct dsh_params {
        is assigned to another averaging cell than in the FP64 run; from then on, cells that take
        the reference's fallback drag (kernel.cu:2160-2211) move the total momentum by
        particle-sized amounts.  The band's time term needs a finite run (t_end < 1e29 or
-       max_steps > 0); without one, 3 + f cells. */
+       max_steps > 0); without one, 3 + f cells.  It looks ahead at most wall_band_horizon
+       steps: further ahead the first membership difference decides the momentum difference
+       anyway (a cfg1 run: the same 2e-5 after 100 steps for speeds 0.5, 1 and 2). */
     int    fp64_wall_band;        /* 1 (default) or 0 */
     double wall_band_speed;       /* band edge speed in units of cs (default 0.5) */
+    int    wall_band_horizon;     /* steps the band's time term covers at most (default 50; 0: the whole run) */
 } dsh_params;
 
 /* data.h structure-of-arrays view; NULL fields are skipped. */

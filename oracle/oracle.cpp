@@ -944,6 +944,7 @@ int orc_tube_params(const dsh_tube_spec* t, dsh_params* p) {
     p->precision = t->precision;
     p->fp64_wall_band = 1;
     p->wall_band_speed = 0.5;
+    p->wall_band_horizon = 50;
     return DSH_OK;
 }
 
