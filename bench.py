@@ -42,7 +42,7 @@ ALGO_BYTES = {
     "idic": (168, 148),
 }
 AVG_CELL_BYTES = 9
-DEFAULT_TILE = 192   # the library's default piece size of the tiled pair kernels
+DEFAULT_SHAPE = (192, 1)   # the library's default block shape: piece size, waves per block
 SORT_STAGES = ("key", "scan", "scatter", "rank_permute", "radius")
 
 
@@ -60,37 +60,38 @@ def stage_bytes(stage, n_gas, n_dust, n_avg):
 
 
 # kernels behind each pass, as named in the rocprofv3 summaries under profiles/
-def pass_kernels(stage, tile=DEFAULT_TILE):
+def pass_kernels(stage, shape=DEFAULT_SHAPE):
     """Kernel names (as rocprofv3 reports them) of one pass of the step."""
+    tile, waves = shape
     return {
         # mixed precision with the FP64 wall band: one fused launch per pair pass
-        "density": [f"k_pairs_fused<0, false, {tile}>"],
-        "force": [f"k_pairs_fused<1, false, {tile}>"],
+        "density": [f"k_pairs_fused<0, false, {tile}, {waves}>"],
+        "force": [f"k_pairs_fused<1, false, {tile}, {waves}>"],
         "idic": ["k_idic_runs<1>"],
         "sort": ["k_key", "k_scan_reduce", "k_scan_top", "k_scan_down", "k_scatter", "k_rank_permute",
                  "k_window_max", "k_radius", "k_band", "k_perm_count", "k_perm_write"],
     }.get(stage, [])
 
 
-def measured_traffic(workload, stage, tile=DEFAULT_TILE):
+def measured_traffic(workload, stage, shape=DEFAULT_SHAPE):
     """HBM bytes per launch of a pass from the committed FETCH_SIZE/WRITE_SIZE summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
     if not os.path.exists(path):
         return None
     kern = json.load(open(path))["kernels"]
-    names = pass_kernels(stage, tile)
+    names = pass_kernels(stage, shape)
     if not names or any(n not in kern for n in names):
         return None
     return sum(kern[n]["read_bytes"] + kern[n]["write_bytes"] for n in names)
 
 
-def measured_valu(workload, stage, tile=DEFAULT_TILE):
+def measured_valu(workload, stage, shape=DEFAULT_SHAPE):
     """VALU issue fraction of a pass's kernel from the committed SQ-counter summary (or None)."""
     path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
     if not os.path.exists(path):
         return None
     kern = json.load(open(path))["kernels"]
-    names = pass_kernels(stage, tile)
+    names = pass_kernels(stage, shape)
     if len(names) != 1 or names[0] not in kern:
         return None
     return kern[names[0]]["valu_issue_frac"]
@@ -184,7 +185,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", type=int, default=1, help="pair kernels: 1 tiled (LDS), 0 direct")
-    ap.add_argument("--no-autotune", action="store_true", help=f"keep the default piece size ({DEFAULT_TILE})")
+    ap.add_argument("--no-autotune", action="store_true", help=f"keep the default block shape (piece size, waves) {DEFAULT_SHAPE}")
     ap.add_argument("--no-sweep", action="store_true", help="leave out the cfg3 sweep object")
     ap.add_argument("--no-serve", action="store_true", help="leave out the cfg4 serving object")
     ap.add_argument("--no-whole-run", action="store_true", help="leave out the whole-run figures")
@@ -205,8 +206,10 @@ def main():
     n_gas, n_dust = sim.counts[0], sim.counts[1]
     tune = None
     if args.variant == 1 and not args.no_autotune:
-        # untimed: each piece size runs 1 + 2 ordinary steps of the tube, the fastest is kept
+        # untimed: each block shape (piece size x waves per block) runs 1 + 2 ordinary steps of
+        # the tube, the fastest is kept
         tune = sim.autotune(steps_per_trial=2)
+    shape = (sim.block_shape()["tile"], sim.block_shape()["waves"])
     sim.step(args.warmup)
     setup_s = time.perf_counter() - t_setup
     st0 = sim.status()
@@ -298,12 +301,12 @@ def main():
                    "autotune": tune},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.config, dominant, tune["tile"] if tune else DEFAULT_TILE)
+                     "traffic": measured_traffic(args.config, dominant, shape)
                      if args.variant == 1 else None,
                      "traffic_source": f"profiles/hbm_traffic_{args.config}.json (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                      "algo_bytes_per_launch": dom_bytes, "avg_ms": dom_ms,
                      # the pair passes are compute-bound: the share of VALU issue slots they use
-                     "valu_issue_frac": measured_valu(args.config, dominant, tune["tile"] if tune else DEFAULT_TILE)
+                     "valu_issue_frac": measured_valu(args.config, dominant, shape)
                      if args.variant == 1 else None,
                      "valu_source": f"profiles/valu_{args.config}.json (rocprofv3 SQ_INSTS_VALU)"},
         "step_roofline": {"algo_bytes": step_bytes, "event_ms": step_ms,

@@ -198,6 +198,10 @@ def _declare(lib):
         "dsh_set_tile": (C.c_int, [vp, C.c_int]),
         "dsh_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_double)]),
         "dsh_batch_set_tile": (C.c_int, [vp, C.c_int]),
+        "dsh_set_block_waves": (C.c_int, [vp, C.c_int]),
+        "dsh_block_shape": (C.c_int, [vp, P(C.c_int)]),
+        "dsh_batch_set_block_waves": (C.c_int, [vp, C.c_int]),
+        "dsh_batch_block_shape": (C.c_int, [vp, P(C.c_int)]),
         "dsh_batch_autotune": (C.c_int, [vp, C.c_int, C.c_int, P(C.c_int), P(C.c_double)]),
         "dsh_set_pair_variant": (C.c_int, [vp, C.c_int]),
         "dsh_neighbour_stats": (C.c_int, [vp, C.c_int, _IP, P(C.c_uint64)]),

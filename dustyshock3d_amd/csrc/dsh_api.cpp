@@ -99,6 +99,19 @@ int dsh_set_tile(dsh_ctx* c, int tile) {
     return DSH_OK;
 }
 
+int dsh_set_block_waves(dsh_ctx* c, int waves) {
+    if (!c || !dsh::valid_block_waves(waves)) return DSH_ERR_ARG;
+    c->eng->set_block_waves(waves);
+    return DSH_OK;
+}
+
+int dsh_block_shape(dsh_ctx* c, int shape[2]) {
+    if (!c || !shape) return DSH_ERR_ARG;
+    shape[0] = c->eng->tile();
+    shape[1] = c->eng->block_waves();
+    return DSH_OK;
+}
+
 int dsh_autotune(dsh_ctx* c, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step) {
     if (!c || steps_per_trial < 1) return DSH_ERR_ARG;
     int tile = 0, variant = 0;

@@ -120,6 +120,19 @@ int dsh_batch_set_tile(dsh_batch* b, int tile) {
     return DSH_OK;
 }
 
+int dsh_batch_set_block_waves(dsh_batch* b, int waves) {
+    if (!b || !dsh::valid_block_waves(waves)) return DSH_ERR_ARG;
+    b->eng->set_block_waves(waves);
+    return DSH_OK;
+}
+
+int dsh_batch_block_shape(dsh_batch* b, int shape[2]) {
+    if (!b || !shape) return DSH_ERR_ARG;
+    shape[0] = b->eng->tile();
+    shape[1] = b->eng->block_waves();
+    return DSH_OK;
+}
+
 int dsh_batch_autotune(dsh_batch* b, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step) {
     if (!b || steps_per_trial < 1) return DSH_ERR_ARG;
     int tile = 0, variant = 0;

@@ -311,6 +311,7 @@ int Engine::prepare() {
     if ((rc = make_map(gseg_, 64, a_.gmap64))) return rc;
     a_.variant = pair_variant_;
     a_.tile = tile_;
+    a_.waves = waves_;
     if ((rc = alloc(&a_.masks, mask_words(a_.gmap64.nblocks)))) return rc;
     a_.pband = nullptr;
     a_.bperm = nullptr;
@@ -623,11 +624,12 @@ int Engine::autotune(int steps_per_trial, bool with_variants, int* tile_out, int
     int rc;
     if (!prepared_ && (rc = prepare())) return rc;
     struct Cand {
-        int tile, variant;
+        int tile, waves, variant;
     };
     std::vector<Cand> cands;
-    for (int t : kTiles) cands.push_back({t, kPairTiled});
-    if (with_variants) cands.push_back({192, kPairDirect});
+    for (int w : kBlockWaves)
+        for (int t : kTiles) cands.push_back({t, w, kPairTiled});
+    if (with_variants) cands.push_back({192, 1, kPairDirect});
     auto finished = [&]() {
         for (const TubeDyn& d : dyn_h_)
             if (!(d.done || d.err)) return false;
@@ -639,10 +641,11 @@ int Engine::autotune(int steps_per_trial, bool with_variants, int* tile_out, int
         (void)hipEventDestroy(e0);
         return rc;
     }
-    Cand best{tile_, pair_variant_};
+    Cand best{tile_, waves_, pair_variant_};
     double best_ms = -1.0;
     for (const Cand& c : cands) {
         set_tile(c.tile);
+        set_block_waves(c.waves);
         set_pair_variant(c.variant);
         if ((rc = run(1))) break;   // builds the graph, warms the caches
         if (finished()) break;
@@ -662,6 +665,7 @@ int Engine::autotune(int steps_per_trial, bool with_variants, int* tile_out, int
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     set_tile(best.tile);
+    set_block_waves(best.waves);
     set_pair_variant(best.variant);
     if (rc) return rc;
     if (tile_out) *tile_out = best.tile;

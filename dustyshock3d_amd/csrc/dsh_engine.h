@@ -77,9 +77,18 @@ public:
         graph_dirty_ = true;
     }
     int tile() const { return tile_; }
-    // Times steps_per_trial steps of each candidate configuration (piece sizes; with
-    // variants also the direct pair kernels, whose sums differ in rounding order) on the
-    // live state and keeps the fastest.  The trial steps are ordinary steps of the run.
+    // waves per block of the tiled pair kernels (kBlockWaves); each wave keeps its own chunk
+    // and LDS tile, so every block shape gives the same results bit for bit
+    void set_block_waves(int w) {
+        waves_ = w;
+        a_.waves = w;
+        graph_dirty_ = true;
+    }
+    int block_waves() const { return waves_; }
+    // Times steps_per_trial steps of each candidate configuration (block shapes: piece size x
+    // waves per block; with variants also the direct pair kernels, whose sums differ in
+    // rounding order) on the live state and keeps the fastest.  The trial steps are ordinary
+    // steps of the run.
     int autotune(int steps_per_trial, bool with_variants, int* tile, int* variant, double* ms_per_step);
     // per-stage GPU time (ms) averaged over the steps launched with timing on
     const std::vector<std::pair<std::string, double>>& kernel_times() const { return ktimes_; }
@@ -116,6 +125,7 @@ private:
     int precision_ = 0;
     int pair_variant_ = kPairTiled;
     int tile_ = 192;
+    int waves_ = 1;
     long long ntot_ = 0;
     size_t dev_bytes_ = 0;
     std::vector<std::pair<std::string, double>> ktimes_;

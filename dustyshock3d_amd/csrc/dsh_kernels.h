@@ -39,6 +39,13 @@ inline bool valid_tile(int t) {
         if (k == t) return true;
     return false;
 }
+// Waves (64-target chunks, each with its own LDS tile) per block of the tiled pair kernels.
+constexpr int kBlockWaves[] = {1, 2, 4};
+inline bool valid_block_waves(int w) {
+    for (int k : kBlockWaves)
+        if (k == w) return true;
+    return false;
+}
 
 struct StepArgs {
     const TubeConst* tc;
@@ -72,6 +79,7 @@ struct StepArgs {
     int* ginv;             // gas slot of each gas index (tubes with drag_mode 0), or null
     int variant;
     int tile;              // candidates staged per piece in the tiled pair kernels (kTiles)
+    int waves;             // waves per block of the tiled pair kernels (kBlockWaves)
     // mixed precision with an FP64 wall band: per sorted particle, 1 if its 64-particle chunk
     // has a particle in its tube's band (written by k_band each step); null otherwise
     unsigned char* pband;

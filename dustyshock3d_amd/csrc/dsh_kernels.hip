@@ -1123,6 +1123,20 @@ struct TileLds {
     int id[kStats ? kTileP : 1];
 };
 
+// The LDS tile of a wave is its own (a block of kWaves waves holds kWaves tiles), so the
+// staging barriers order one wave's LDS writes and reads only: waves of a block never wait
+// for each other, and a wave may leave early.  One wave per block: the block barrier.
+template <int kWaves>
+__device__ __forceinline__ void tile_sync() {
+    if constexpr (kWaves == 1) {
+        __syncthreads();
+    } else {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+}
+
 // KIND 0: density (all species); KIND 1: gas force and de/dt.
 // The gas density pass stores its predicate hit masks (one 64-bit word per lane per
 // uniform (row, piece, window) iteration, kMaskSlots per chunk); the force pass replays
@@ -1134,7 +1148,7 @@ struct TileLds {
 // one wave.  With pband (mixed precision with an FP64 wall band), a chunk whose band flag does
 // not match the instance's precision is left to the other instance; the force pair of two
 // band particles is FP64, of a frozen and an active particle FP64, of any other two FP32.
-template <typename R, int KIND, bool kStats, int kTileP>
+template <typename R, int KIND, bool kStats, int kTileP, int kWaves>
 __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L, int b, const Map& mp,
                                             const TubeConst* __restrict__ tc,
                                             const TubeDyn* __restrict__ td, const int* __restrict__ cur,
@@ -1148,7 +1162,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     static_assert(kTileP >= 64, "a density window of 64 slots fits in one piece");
     const int sgi = find_seg(mp, b);
     const Seg sg = mp.seg[sgi];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int li = (b - mp.blk[sgi]) * 64 + lane;
     const TubeDyn& D = td[sg.tube];
     if (D.done || D.err) return;
@@ -1244,7 +1258,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     // All loads of a candidate are issued before its LDS writes (the scheduling barrier
     // keeps the compiler from serialising them under register pressure).
     auto stage = [&](int s0, int n0, int s1, int n1) {
-        __syncthreads();
+        tile_sync<kWaves>();
         if constexpr (KIND == 1) {
             if (use_masks && segi < kMaskSlots) {
                 pw_at = segi;
@@ -1323,7 +1337,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             if (kStats) L.id[k] = id;
         }
         if constexpr (kFix) far_piece = far_images || __ballot(out_of_range) != 0ull;
-        __syncthreads();
+        tile_sync<kWaves>();
     };
     // Hit mask of one window of 64 candidates of this lane's range, starting at staged slot
     // a (>= 0); len = candidates of the lane's range from a on, wn = the wave's maximum of
@@ -1803,17 +1817,21 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     }
 }
 
-// Every chunk of the map, one per block (blocks remapped so each XCD takes a contiguous run).
-template <typename R, int KIND, bool kStats, int kTileP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(
+// Every chunk of the map, kWaves per block (wave w of block b takes chunk b * kWaves + w;
+// blocks remapped so each XCD takes a contiguous run).
+template <typename R, int KIND, bool kStats, int kTileP, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(
     Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
     const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
     double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
     unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
     const unsigned char* __restrict__ pband, ForceRec* __restrict__ frec) {
-    __shared__ TileLds<R, KIND, kStats, kTileP> L;
-    pairs_chunk<R, KIND, kStats, kTileP>(L, xcd_remap(blockIdx.x, gridDim.x), mp, tc, td, cur, B, start, wkey, reach,
-                                         pr2, csav, ncount, nhash, masks, mask_chunks, pband, frec);
+    __shared__ TileLds<R, KIND, kStats, kTileP> L[kWaves];
+    const int wave = kWaves == 1 ? 0 : uni((int)(threadIdx.x >> 6));
+    const int b = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
+    if (kWaves > 1 && b >= mp.nblocks) return;
+    pairs_chunk<R, KIND, kStats, kTileP, kWaves>(L[wave], b, mp, tc, td, cur, B, start, wkey, reach, pr2, csav,
+                                                 ncount, nhash, masks, mask_chunks, pband, frec);
 }
 
 // Block b of the part [p0, p1) of a grid, remapped so that each XCD (which takes the blocks
@@ -1849,9 +1867,12 @@ constexpr int band_tile() {
 // are dispatched first, so the band's slower waves overlap the FP32 ones instead of trailing
 // them; the rest run the FP32 path.  For the gas force map (KIND 1) the band part is the gas
 // band chunks (perm[nall + 1] of them) and the FP32 part starts at perm[nband_all].  The two
-// paths share one LDS allocation.
-template <int KIND, bool kStats, int kTileP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_fused(
+// paths share one LDS tile per wave.  With kWaves waves per block, the band part takes
+// ceil(nband / kWaves) whole blocks (wave w of part block p: band chunk p * kWaves + w) and
+// the FP32 part the blocks after them; waves past a part's end leave at once.  The grid is
+// ceil(nchunks / kWaves) + 1 blocks, enough for both parts whatever the band count.
+template <int KIND, bool kStats, int kTileP, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_fused(
     Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
     const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
     double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
@@ -1860,21 +1881,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 
     constexpr int kBandTile = band_tile<kTileP>();
     using L32 = TileLds<float, KIND, kStats, kTileP>;
     using L64 = TileLds<double, KIND, kStats, kBandTile>;
-    __shared__ __align__(16) unsigned char raw[sizeof(L32) > sizeof(L64) ? sizeof(L32) : sizeof(L64)];
+    constexpr size_t kTileBytes = ((sizeof(L32) > sizeof(L64) ? sizeof(L32) : sizeof(L64)) + 15) / 16 * 16;
+    __shared__ __align__(16) unsigned char raw[kWaves][kTileBytes];
+    const int wave = kWaves == 1 ? 0 : uni((int)(threadIdx.x >> 6));
     const int nb = gridDim.x;
     const int band_all = uni(perm[nall]);
     const int nband = KIND == 0 ? band_all : uni(perm[nall + 1]);
+    const int nrest = mp.nblocks - nband;
+    const int bb = (nband + kWaves - 1) / kWaves;
     const int b = blockIdx.x;
-    if (b < nband) {
-        const int e = part_remap(b, 0, nband);
-        pairs_chunk<double, KIND, kStats, kBandTile>(*reinterpret_cast<L64*>(raw), uni(perm[e]), mp, tc, td, cur, B,
-                                                     start, wkey, reach, pr2, csav, ncount, nhash, masks, mask_chunks,
-                                                     pband, frec);
+    if (b < bb) {
+        const int e = part_remap(b, 0, bb) * kWaves + wave;
+        if (kWaves > 1 && e >= nband) return;
+        pairs_chunk<double, KIND, kStats, kBandTile, kWaves>(*reinterpret_cast<L64*>(raw[wave]), uni(perm[e]), mp, tc,
+                                                             td, cur, B, start, wkey, reach, pr2, csav, ncount, nhash,
+                                                             masks, mask_chunks, pband, frec);
     } else {
-        const int e = part_remap(b, nband, nb);
-        pairs_chunk<float, KIND, kStats, kTileP>(*reinterpret_cast<L32*>(raw), uni(perm[band_all + (e - nband)]), mp,
-                                                 tc, td, cur, B, start, wkey, reach, pr2, csav, ncount, nhash, masks,
-                                                 mask_chunks, pband, frec);
+        const int e = (part_remap(b, bb, nb) - bb) * kWaves + wave;
+        if (e >= nrest) return;
+        pairs_chunk<float, KIND, kStats, kTileP, kWaves>(*reinterpret_cast<L32*>(raw[wave]), uni(perm[band_all + e]),
+                                                         mp, tc, td, cur, B, start, wkey, reach, pr2, csav, ncount,
+                                                         nhash, masks, mask_chunks, pband, frec);
     }
 }
 
@@ -2462,40 +2489,54 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <typename R, int KIND, bool kStats, int kTileP>
+template <typename R, int KIND, bool kStats, int kTileP, int kWaves>
 static void launch_tiled_t(const StepArgs& a, const Map& m, const unsigned char* pband, hipStream_t s) {
-    hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
-                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
-                       pband, a.frec);
+    const int grid = (m.nblocks + kWaves - 1) / kWaves;
+    hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP, kWaves>), dim3(grid), dim3(64 * kWaves), 0, s, m, a.tc,
+                       a.td, a.cur, a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks,
+                       a.gmap64.nblocks, pband, a.frec);
 }
 
-template <int KIND, bool kStats, int kTileP>
+template <int KIND, bool kStats, int kTileP, int kWaves>
 static void launch_fused_t(const StepArgs& a, const Map& m, hipStream_t s) {
-    hipLaunchKernelGGL((k_pairs_fused<KIND, kStats, kTileP>), dim3(m.nblocks), dim3(64), 0, s, m, a.tc, a.td, a.cur,
-                       a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks, a.gmap64.nblocks,
-                       a.pband, a.bperm, a.pmap64.nblocks, a.frec);
+    // both parts in whole blocks: ceil(nband / W) + ceil((n - nband) / W) <= ceil(n / W) + 1
+    const int grid = kWaves == 1 ? m.nblocks : (m.nblocks + kWaves - 1) / kWaves + 1;
+    hipLaunchKernelGGL((k_pairs_fused<KIND, kStats, kTileP, kWaves>), dim3(grid), dim3(64 * kWaves), 0, s, m, a.tc,
+                       a.td, a.cur, a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks,
+                       a.gmap64.nblocks, a.pband, a.bperm, a.pmap64.nblocks, a.frec);
+}
+
+// Calls f with the block shape (piece size, waves per block) of a as compile-time constants.
+template <typename F>
+static void with_shape(const StepArgs& a, F&& f) {
+    auto waves = [&](auto tile) {
+        switch (a.waves) {
+            case 2: f(tile, std::integral_constant<int, 2>{}); break;
+            case 4: f(tile, std::integral_constant<int, 4>{}); break;
+            default: f(tile, std::integral_constant<int, 1>{}); break;
+        }
+    };
+    switch (a.tile) {
+        case 128: waves(std::integral_constant<int, 128>{}); break;
+        case 192: waves(std::integral_constant<int, 192>{}); break;
+        case 224: waves(std::integral_constant<int, 224>{}); break;
+        case 256: waves(std::integral_constant<int, 256>{}); break;
+        default: waves(std::integral_constant<int, 160>{}); break;
+    }
 }
 
 template <int KIND, bool kStats>
 static void launch_fused(const StepArgs& a, const Map& m, hipStream_t s) {
-    switch (a.tile) {
-        case 128: launch_fused_t<KIND, kStats, 128>(a, m, s); break;
-        case 192: launch_fused_t<KIND, kStats, 192>(a, m, s); break;
-        case 224: launch_fused_t<KIND, kStats, 224>(a, m, s); break;
-        case 256: launch_fused_t<KIND, kStats, 256>(a, m, s); break;
-        default: launch_fused_t<KIND, kStats, 160>(a, m, s); break;
-    }
+    with_shape(a, [&](auto tile, auto waves) {
+        launch_fused_t<KIND, kStats, decltype(tile)::value, decltype(waves)::value>(a, m, s);
+    });
 }
 
 template <typename R, int KIND, bool kStats>
 static void launch_tiled1(const StepArgs& a, const Map& m, const unsigned char* pband, hipStream_t s) {
-    switch (a.tile) {
-        case 128: launch_tiled_t<R, KIND, kStats, 128>(a, m, pband, s); break;
-        case 192: launch_tiled_t<R, KIND, kStats, 192>(a, m, pband, s); break;
-        case 224: launch_tiled_t<R, KIND, kStats, 224>(a, m, pband, s); break;
-        case 256: launch_tiled_t<R, KIND, kStats, 256>(a, m, pband, s); break;
-        default: launch_tiled_t<R, KIND, kStats, 160>(a, m, pband, s); break;
-    }
+    with_shape(a, [&](auto tile, auto waves) {
+        launch_tiled_t<R, KIND, kStats, decltype(tile)::value, decltype(waves)::value>(a, m, pband, s);
+    });
 }
 
 // FP64 mode: the FP64 instance over every chunk.  Mixed without a wall band: the FP32

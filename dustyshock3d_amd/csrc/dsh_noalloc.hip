@@ -280,6 +280,7 @@ int step_noalloc(const dsh_params* P, const dsh_soa_view* in, const int* counts,
     a.idic_runs = C.f == 1 ? 1u << (nsp - 1) : 0u;
     a.variant = kPairTiled;
     a.tile = 192;
+    a.waves = 1;
 
     // set-up: records, cleared state and cells, inputs in buffer 0 in original order
     const int zero2[2] = {0, 0};

@@ -160,12 +160,23 @@ class Simulation:
         """Candidates staged per LDS piece by the tiled pair kernels (128, 160, 192, 224, 256)."""
         self._call(self._lib.dsh_set_tile(self._ctx, int(tile)))
 
+    def set_block_waves(self, waves: int):
+        """Waves (64-target chunks) per block of the tiled pair kernels (1, 2, 4)."""
+        self._call(self._lib.dsh_set_block_waves(self._ctx, int(waves)))
+
+    def block_shape(self) -> dict:
+        shape = (C.c_int * 2)()
+        self._call(self._lib.dsh_block_shape(self._ctx, shape))
+        return {"tile": int(shape[0]), "waves": int(shape[1])}
+
     def autotune(self, steps_per_trial: int = 2, with_variants: bool = False) -> dict:
-        """Time each pair-kernel configuration on the live state and keep the fastest."""
+        """Time each pair-kernel configuration (piece size x waves per block) on the live
+        state and keep the fastest."""
         chosen = (C.c_int * 2)()
         ms = C.c_double(0.0)
         self._call(self._lib.dsh_autotune(self._ctx, int(steps_per_trial), int(with_variants), chosen, C.byref(ms)))
-        return {"tile": int(chosen[0]), "variant": int(chosen[1]), "ms_per_step": ms.value}
+        return {"tile": int(chosen[0]), "waves": self.block_shape()["waves"], "variant": int(chosen[1]),
+                "ms_per_step": ms.value}
 
     def set_pair_variant(self, variant: int):
         self._call(self._lib.dsh_set_pair_variant(self._ctx, int(variant)))

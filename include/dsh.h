@@ -201,10 +201,17 @@ int dsh_set_pair_variant(dsh_ctx* ctx, int variant);
 /* Candidates staged per LDS piece by the tiled pair kernels: 128, 160, 192 (default), 224 or 256.
    Pieces split each candidate row in order, so every size gives the same results bit for bit. */
 int dsh_set_tile(dsh_ctx* ctx, int tile);
-/* Autotuner (no reference counterpart): runs steps_per_trial steps with each piece size
-   (and, with with_variants, the direct pair kernels, whose sums round in another order)
-   on the live state, keeps the fastest and returns chosen[0] = tile, chosen[1] = variant
-   and its time per step.  The trial steps are ordinary steps of the run. */
+/* Waves per block of the tiled pair kernels: 1 (default), 2 or 4 (no reference counterpart;
+   the reference launches fixed blocks, kernel.cu:1867-1868).  Each wave keeps its own
+   64-target chunk and LDS piece, so every block shape gives the same results bit for bit. */
+int dsh_set_block_waves(dsh_ctx* ctx, int waves);
+/* The block shape in use: shape[0] = piece size, shape[1] = waves per block. */
+int dsh_block_shape(dsh_ctx* ctx, int shape[2]);
+/* Autotuner (no reference counterpart): runs steps_per_trial steps with each block shape
+   (piece size x waves per block; with with_variants also the direct pair kernels, whose
+   sums round in another order) on the live state, keeps the fastest and returns
+   chosen[0] = tile, chosen[1] = variant and its time per step (dsh_block_shape gives the
+   chosen waves).  The trial steps are ordinary steps of the run. */
 int dsh_autotune(dsh_ctx* ctx, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step);
 int dsh_neighbour_stats(dsh_ctx* ctx, int species, int* count, uint64_t* hash);
 /* Particle indices in the device's sorted order of the last step: by neighbour cell, inside a
@@ -248,6 +255,8 @@ int dsh_batch_download(dsh_batch* b, int tube, int species, const dsh_soa_view* 
 int dsh_batch_conservation(dsh_batch* b, int tube, double out[4]);
 int dsh_batch_set_pair_variant(dsh_batch* b, int variant);
 int dsh_batch_set_tile(dsh_batch* b, int tile);
+int dsh_batch_set_block_waves(dsh_batch* b, int waves);
+int dsh_batch_block_shape(dsh_batch* b, int shape[2]);
 int dsh_batch_autotune(dsh_batch* b, int steps_per_trial, int with_variants, int chosen[2], double* ms_per_step);
 int dsh_batch_set_timing(dsh_batch* b, int on);
 int dsh_batch_kernel_times(dsh_batch* b, double* ms, char* names, int max_kernels, int name_len);
