@@ -2053,15 +2053,24 @@ __device__ __forceinline__ CellDrag cell_drag(const TubeConst& T, int nd, double
 __device__ __forceinline__ void particle_update(const TubeConst& T, const PBuf& B, long long g, int s, int nd,
                                                 double tau, const CellDrag& C, double& px, double& py, double& pz,
                                                 double& E, double& mn) {
+    // every input is read before the first store: the stores below could alias any field as
+    // far as the compiler knows, so loads left after them would each wait for a round trip
+    const bool gas_kick = s == 0 && nd == 0;
     double vx = B.vx[g], vy = B.vy[g], vz = B.vz[g];
     const double m = B.m[g];
+    const bool active = B.ind[g] == 0;
+    const double ax = gas_kick ? B.ax[g] : 0.0, ay = gas_kick ? B.ay[g] : 0.0, az = gas_kick ? B.az[g] : 0.0;
+    const double x0 = B.x[g], y0 = B.y[g], z0 = B.z[g];
+    const double e0 = s == 0 ? B.e[g] : 0.0, ed = s == 0 ? B.ed[g] : 0.0;
+    const double h0 = B.h[g];
+    const double rho = T.h_mode == 1 ? B.rho[g] : 1.0;
     double e = 0.0;
-    if (B.ind[g] == 0) {
-        if (s == 0 && nd == 0) {
+    if (active) {
+        if (gas_kick) {
             // gas only: v += A tau (MoveKernel, kernel.cu:205-228)
-            vx = vx + B.ax[g] * tau;
-            vy = vy + B.ay[g] * tau;
-            vz = vz + B.az[g] * tau;
+            vx = vx + ax * tau;
+            vy = vy + ay * tau;
+            vz = vz + az * tau;
         } else if (s == 0) {
             vx = (vx / tau + C.tmpv[0] + C.psi[0]) / (1.0 / tau + C.tmp1);
             vy = (vy / tau + C.tmpv[1] + C.psi[1]) / (1.0 / tau + C.tmp1);
@@ -2075,27 +2084,27 @@ __device__ __forceinline__ void particle_update(const TubeConst& T, const PBuf& 
         B.vx[g] = vx;
         B.vy[g] = vy;
         B.vz[g] = vz;
-        double x = B.x[g] + vx * tau;
-        double y = B.y[g] + vy * tau;
-        double z = B.z[g] + vz * tau;
+        double x = x0 + vx * tau;
+        double y = y0 + vy * tau;
+        double z = z0 + vz * tau;
         if (T.per[1]) y = wrap(y, T.o[1], T.L[1], T.hi[1]);
         if (T.per[2]) z = wrap(z, T.o[2], T.L[2], T.hi[2]);
         B.x[g] = x;
         B.y[g] = y;
         B.z[g] = z;
         if (s == 0) {
-            e = B.e[g] + B.ed[g] * tau;
+            e = e0 + ed * tau;
             B.e[g] = e;
         }
-        double h = B.h[g];
+        double h = h0;
         if (T.h_mode == 1) {
-            h = T.eta * cbrt(m / B.rho[g]);
+            h = T.eta * cbrt(m / rho);
             B.h[g] = h;
         }
         const double v = sqrt(vx * vx + vy * vy + vz * vz);
         mn = fmin(mn, h / (T.cs + v));
     } else if (s == 0) {
-        e = B.e[g];
+        e = e0;
     }
     px += m * vx;
     py += m * vy;
@@ -2203,7 +2212,7 @@ __global__ __launch_bounds__(kBS) void k_idic(Map mp, const TubeConst* __restric
 // parameter (one instance per count present), so registers hold only the sums in use.
 constexpr int kRunCells = 8;
 template <int kNd>
-__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(8))) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
+__global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(6))) void k_idic_runs(Map mp, const TubeConst* __restrict__ tc,
                                                    const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
                                                    const int* __restrict__ start, const int* __restrict__ ginv,
                                                    double* __restrict__ part) {
