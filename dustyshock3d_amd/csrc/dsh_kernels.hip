@@ -601,34 +601,49 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
     }
     const int dst = lo + r;
     const long long wo = (long long)(1 - *cur) * B.ntot;
-    B.x[wo + dst] = B.x[so + src];
-    B.y[wo + dst] = B.y[so + src];
-    B.z[wo + dst] = B.z[so + src];
-    B.vx[wo + dst] = B.vx[so + src];
-    B.vy[wo + dst] = B.vy[so + src];
-    B.vz[wo + dst] = B.vz[so + src];
-    B.m[wo + dst] = B.m[so + src];
-    hq = B.h[so + src];
-    B.h[wo + dst] = hq;
-    const int ind = B.ind[so + src];
-    B.ind[wo + dst] = ind;
-    B.id[wo + dst] = my;
-    // density, pressure and stopping time are recomputed this step for active particles of
-    // running tubes; frozen particles and finished tubes carry theirs along
-    const bool keep = ind != 0 || td[ip.sg.tube].done || td[ip.sg.tube].err;
-    if (keep) B.rho[wo + dst] = B.rho[so + src];
-    if (ip.sg.species == 0) {
+    const long long gs = so + src, gd = wo + dst;
+    // every field is read before the first store: a store could alias any field as far as
+    // the compiler knows, so loads left after stores would each wait for a round trip.
+    // Density, pressure and stopping time are recomputed this step for active particles of
+    // running tubes; frozen particles and finished tubes carry theirs along.
+    const bool gas = ip.sg.species == 0;
+    const bool done = td[ip.sg.tube].done != 0;
+    const bool tube_keep = done || td[ip.sg.tube].err;
+    const double x = B.x[gs], y = B.y[gs], z = B.z[gs];
+    const double vx = B.vx[gs], vy = B.vy[gs], vz = B.vz[gs];
+    const double m = B.m[gs];
+    hq = B.h[gs];
+    const int ind = B.ind[gs];
+    const bool keep = ind != 0 || tube_keep;
+    const double rho = keep ? B.rho[gs] : 0.0;
+    const double e = gas ? B.e[gs] : 0.0;
+    const double pr = gas && keep ? B.p[gs] : 0.0;
+    const double ts = !gas && keep ? B.ts[gs] : 0.0;
+    const double ax = gas && done ? B.ax[gs] : 0.0, ay = gas && done ? B.ay[gs] : 0.0;
+    const double az = gas && done ? B.az[gs] : 0.0, ed = gas && done ? B.ed[gs] : 0.0;
+    B.x[gd] = x;
+    B.y[gd] = y;
+    B.z[gd] = z;
+    B.vx[gd] = vx;
+    B.vy[gd] = vy;
+    B.vz[gd] = vz;
+    B.m[gd] = m;
+    B.h[gd] = hq;
+    B.ind[gd] = ind;
+    B.id[gd] = my;
+    if (keep) B.rho[gd] = rho;
+    if (gas) {
         if (ginv) ginv[ip.sg.off + my] = dst;
-        B.e[wo + dst] = B.e[so + src];
-        if (keep) B.p[wo + dst] = B.p[so + src];
-        if (td[ip.sg.tube].done) {
-            B.ax[wo + dst] = B.ax[so + src];
-            B.ay[wo + dst] = B.ay[so + src];
-            B.az[wo + dst] = B.az[so + src];
-            B.ed[wo + dst] = B.ed[so + src];
+        B.e[gd] = e;
+        if (keep) B.p[gd] = pr;
+        if (done) {
+            B.ax[gd] = ax;
+            B.ay[gd] = ay;
+            B.az[gd] = az;
+            B.ed[gd] = ed;
         }
     } else if (keep) {
-        B.ts[wo + dst] = B.ts[so + src];
+        B.ts[gd] = ts;
     }
     wkey[dst] = c;
     }
