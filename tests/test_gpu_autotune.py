@@ -30,8 +30,9 @@ def test_piece_sizes_agree_bit_for_bit(precision):
                 assert np.array_equal(runs[tile][sp][k], runs[TILES[0]][sp][k]), (tile, sp, k)
 
 
-# (piece size, waves per block); the first is the reference shape
-SHAPES = ((192, 1), (192, 2), (192, 4), (128, 4), (256, 2), (160, 4))
+# (piece size, waves per block); the first is the reference shape, the last the largest LDS
+# footprint (FP64 force with neighbour statistics: four 256-candidate tiles, ~95 KB per block)
+SHAPES = ((192, 1), (192, 2), (192, 4), (128, 4), (256, 2), (160, 4), (256, 4))
 
 
 def _run_shape(spec, band, tile, waves, steps):
