@@ -31,7 +31,7 @@ def test_piece_sizes_agree_bit_for_bit(precision):
 
 
 # (piece size, waves per block); the first is the reference shape, the last the largest LDS
-# footprint (FP64 force with neighbour statistics: four 256-candidate tiles, ~95 KB per block)
+# footprint (the FP64 force kernel: four 256-candidate tiles, ~91 KB per block)
 SHAPES = ((192, 1), (192, 2), (192, 4), (128, 4), (256, 2), (160, 4), (256, 4))
 
 
