@@ -1074,6 +1074,12 @@ __device__ __forceinline__ int wave_max_u9(int v) {
 }
 // a value equal on every lane, as a scalar (kept in SGPRs)
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uni(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // One candidate of the FP64 neighbour predicate, as a bit at position t (t < 32, uniform).
 // One candidate's position and smoothing length, gathered from the staged arrays.
@@ -1723,6 +1729,10 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
         const int cxa = __builtin_amdgcn_readlane(kx, __ffsll((long long)mm) - 1);
         const int cxb = __builtin_amdgcn_readlane(kx, 63 - __builtin_clzll(mm));
         const double rmax = wave_max(mine ? rch : 0.0);
+        // x extent of this row's targets: a candidate with x_j - xmin <= -rmax or
+        // x_j - xmax >= rmax (as the window searches round them) is in no lane's x-window
+        const double xmin = uni(wave_min(mine ? xi : __builtin_inf()));
+        const double xmax = uni(wave_max(mine ? xi : -__builtin_inf()));
         const int Rx = uni((int)ceil(rmax * T.ic[0]));
         const int Ry = uni((int)ceil(rmax * T.ic[1]));
         const int Rz = uni((int)ceil(rmax * T.ic[2]));
@@ -1757,6 +1767,20 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             g.shifted = (sy != 0) || (sz != 0);
             g.ub = uni(start[rowbase + max(cxa - Rx, 0)]);
             g.ue = uni(start[rowbase + min(cxb + Rx, nx - 1) + 1]);
+            if (g.ue > g.ub) {
+                // Trim the row's ends (whole cells of the reach) to the candidates some lane
+                // can reach, up to 64 at each end: x ascends along the row, so the
+                // unreachable candidates at each end are a prefix / suffix, counted by ballot.
+                // Smaller rows let more mirror pairs share a piece; the sets and the order of
+                // every target's hits are unchanged (only candidates outside every window go).
+                const int klo = g.ub + lane, khi = g.ue - 64 + lane;
+                const bool vlo = klo < g.ue, vhi = khi >= g.ub;
+                const double xl = B.x[wo + (vlo ? klo : g.ub)], xh = B.x[wo + (vhi ? khi : g.ub)];
+                const int cut_lo = __popcll(__ballot(vlo && xl - xmin <= -rmax));
+                const int cut_hi = __popcll(__ballot(vhi && xh - xmax >= rmax));
+                g.ub += cut_lo;
+                g.ue -= cut_hi;
+            }
             g.jl = 0;
             g.jh = 0;
             if (need) {
