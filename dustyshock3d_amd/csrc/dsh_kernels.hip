@@ -2285,21 +2285,30 @@ __global__ __launch_bounds__(kBS) __attribute__((amdgpu_waves_per_eu(6))) void k
                 if (s == 0) S.ng = jhi - jlo;
                 else S.ndc[s - 1] = jhi - jlo;
             }
-            for (int base = p0; base < p1; base += 64) {
+            // the next 64 particles' values are read while the current ones are summed
+            double nv[6] = {0, 0, 0, 0, 0, 0};
+            auto fetch = [&](int base) {
                 const int k = base + lane;
                 if (k < p1) {
                     const long long g = wo + k;
-                    V[0][lane] = B.vx[g];
-                    V[1][lane] = B.vy[g];
-                    V[2][lane] = B.vz[g];
+                    nv[0] = B.vx[g];
+                    nv[1] = B.vy[g];
+                    nv[2] = B.vz[g];
                     if (s == 0) {
-                        V[3][lane] = B.ax[g];
-                        V[4][lane] = B.ay[g];
-                        V[5][lane] = B.az[g];
+                        nv[3] = B.ax[g];
+                        nv[4] = B.ay[g];
+                        nv[5] = B.az[g];
                     } else {
-                        V[3][lane] = dust_tstop(T, B, ginv, wo, k, s);
+                        nv[3] = dust_tstop(T, B, ginv, wo, k, s);
                     }
                 }
+            };
+            if (p0 < p1) fetch(p0);
+            for (int base = p0; base < p1; base += 64) {
+                if (base + lane < p1) {
+                    for (int c = 0; c < (s == 0 ? 6 : 4); ++c) V[c][lane] = nv[c];
+                }
+                if (base + 64 < p1) fetch(base + 64);
                 __builtin_amdgcn_wave_barrier();
                 if (lane < ncl) {
                     const int a = max(jlo, base) - base, e = min(jhi, base + 64) - base;
