@@ -1137,8 +1137,11 @@ struct TileLds {
     HT h[kTileP];
     R vx[KIND ? kTileP : 1], vy[KIND ? kTileP : 1], vz[KIND ? kTileP : 1];
     R m[KIND ? kTileP : 1];
-    R mw[KIND ? 1 : kTileP];    // density: m / pi
-    R hh[KIND ? 1 : kTileP];    // density: h / 2
+    // density: h / 2 and m / pi side by side, one LDS read per hit at the stride of x
+    struct alignas(2 * sizeof(R)) HM {
+        R hh, mw;
+    };
+    HM hm[KIND ? 1 : kTileP];
     R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
     unsigned char ind[KIND ? kTileP : 1];
     int id[kStats ? kTileP : 1];
@@ -1342,8 +1345,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                 L.h[k] = h;
             }
             if (KIND == 0) {
-                L.mw[k] = (R)(m * kInvPi);
-                L.hh[k] = (R)(0.5 * h);
+                L.hm[k] = {(R)(0.5 * h), (R)(m * kInvPi)};
             } else {
                 L.m[k] = (R)m;
                 L.vx[k] = (R)vx;
@@ -1522,18 +1524,25 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             }
             return;
         }
-        const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
-        const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
-        const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        if (KIND == 0) {
+        if constexpr (KIND == 0) {
+            // every LDS read of the hit is issued before the arithmetic waits on any
+            const auto hm = L.hm[u];
+            const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
+            const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+            const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
             const R r = sqrt_r<R>((R)dd);
-            const R ih = rcp<R>(hhi + L.hh[u]);
-            a0.add(L.mw[u] * (ih * ih * ih) * spline_w_hit<R>(r * ih));
+            const R ih = rcp<R>(hhi + hm.hh);
+            a0.add(hm.mw * (ih * ih * ih) * spline_w_hit<R>(r * ih));
             if (kStats) {
                 ++cnt;
                 hash += pair_tag(L.id[u], sy, sz);
             }
-        } else if (dd > 0.0) {
+            return;
+        }
+        const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
+        const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+        const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        if (dd > 0.0) {
             const double hhd = 0.5 * (hi + L.h[u]);
             if (sizeof(R) == 4 && (L.ind[u] & 1) != 0) {
                 // a frozen neighbour has no partner term: FP64 keeps the momentum it
