@@ -1404,7 +1404,12 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             const int base = min(a, kTileP - ((wn + 3) & ~3));
             const int shift = a - base;
             // candidates are shifted in from the top, so candidate t lands on bit t
-            const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + base, *ch_ = L.h + base;
+            // z and h are addressed from a register of their own: one ds_read2 offset cannot
+            // reach them from x's address, and without it the compiler adds two base offsets
+            // per pair of slots (the empty asm keeps it from folding zbase back into base)
+            int zbase = base;
+            asm volatile("" : "+v"(zbase));
+            const double *cx_ = L.x + base, *cy_ = L.y + base, *cz_ = L.z + zbase, *ch_ = L.h + zbase;
             // whole groups of 4 (slots past a lane's window are masked)
             const int wn4 = (wn + 3) & ~3;
             const int n0 = min(wn4, 32);
