@@ -1128,10 +1128,10 @@ __device__ __forceinline__ int fix_diff(int a, int b, int images) {
 // record lets larger pieces (more mirror-row pairs) keep 4 waves per SIMD.
 template <typename R, int KIND>
 constexpr bool kFixYZ = KIND == 1 && sizeof(R) == 4;
-template <typename R, int KIND, bool kStats, int kTileP>
+template <typename R, int KIND, bool kStats, int kTileP, bool kPairs = kFixYZ<R, KIND>>
 struct TileLds {
-    using YZ = std::conditional_t<kFixYZ<R, KIND>, int, double>;
-    using HT = std::conditional_t<kFixYZ<R, KIND>, float, double>;
+    using YZ = double;
+    using HT = double;
     double x[kTileP];
     YZ y[kTileP], z[kTileP];
     HT h[kTileP];
@@ -1145,6 +1145,56 @@ struct TileLds {
     R rho[KIND ? kTileP : 1], pr2[KIND ? kTileP : 1], cs[KIND ? kTileP : 1];
     unsigned char ind[KIND ? kTileP : 1];
     int id[kStats ? kTileP : 1];
+    __device__ YZ Y(int u) const { return y[u]; }
+    __device__ YZ Z(int u) const { return z[u]; }
+    __device__ HT H(int u) const { return h[u]; }
+    __device__ R M(int u) const { return m[u]; }
+    __device__ R VX(int u) const { return vx[u]; }
+    __device__ R VY(int u) const { return vy[u]; }
+    __device__ R VZ(int u) const { return vz[u]; }
+    __device__ R RHO(int u) const { return rho[u]; }
+    __device__ R PR2(int u) const { return pr2[u]; }
+    __device__ R CS(int u) const { return cs[u]; }
+};
+// The FP32 force tile: every 4-byte field in 8-byte pairs, so that all of a candidate's
+// fields sit at multiples of its index times 8 (one address for x and every pair, no
+// address arithmetic per field) and each pair is one LDS access.
+template <typename R, int KIND, bool kStats, int kTileP>
+struct TileLds<R, KIND, kStats, kTileP, true> {
+    using YZ = int;
+    using HT = float;
+    struct alignas(8) I2 {
+        int a, b;
+    };
+    struct alignas(8) F2 {
+        float a, b;
+    };
+    double x[kTileP];
+    I2 yz[kTileP];    // y, z (fixed point)
+    F2 hm[kTileP];    // h, m
+    F2 vxy[kTileP];   // vx, vy
+    F2 vzr[kTileP];   // vz, rho
+    F2 pc[kTileP];    // pr2, cs
+    unsigned char ind[kTileP];
+    int id[kStats ? kTileP : 1];
+    __device__ int Y(int u) const { return yz[u].a; }
+    __device__ int Z(int u) const { return yz[u].b; }
+    __device__ float H(int u) const { return hm[u].a; }
+    __device__ float M(int u) const { return hm[u].b; }
+    __device__ float VX(int u) const { return vxy[u].a; }
+    __device__ float VY(int u) const { return vxy[u].b; }
+    __device__ float VZ(int u) const { return vzr[u].a; }
+    __device__ float RHO(int u) const { return vzr[u].b; }
+    __device__ float PR2(int u) const { return pc[u].a; }
+    __device__ float CS(int u) const { return pc[u].b; }
+    __device__ void set(int k, int y, int z, float h, float m, float vx, float vy, float vz, float rho, float pr2,
+                        float cs) {
+        yz[k] = I2{y, z};
+        hm[k] = F2{h, m};
+        vxy[k] = F2{vx, vy};
+        vzr[k] = F2{vz, rho};
+        pc[k] = F2{pr2, cs};
+    }
 };
 
 // The LDS tile of a wave is its own (a block of kWaves waves holds kWaves tiles), so the
@@ -1303,16 +1353,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                     const ForceRec r = frec[src];
                     __builtin_amdgcn_sched_barrier(0);
                     L.x[k] = r.x;
-                    L.y[k] = r.y;
-                    L.z[k] = r.z;
-                    L.h[k] = r.h;
-                    L.m[k] = r.m;
-                    L.vx[k] = r.vx;
-                    L.vy[k] = r.vy;
-                    L.vz[k] = r.vz;
-                    L.rho[k] = r.rho;
-                    L.pr2[k] = r.pr2;
-                    L.cs[k] = r.cs;
+                    L.set(k, r.y, r.z, r.h, r.m, r.vx, r.vy, r.vz, r.rho, r.pr2, r.cs);
                     L.ind[k] = (unsigned char)(r.flags & 3u);
                     out_of_range = out_of_range || (r.flags & 4u) == 0u;
                     continue;
@@ -1335,27 +1376,28 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             __builtin_amdgcn_sched_barrier(0);
             L.x[k] = x;
             if constexpr (kFix) {
-                L.y[k] = fix_coord(y, T.o[1], ups_y);
-                L.z[k] = fix_coord(z, T.o[2], ups_z);
-                L.h[k] = (float)h;
+                L.set(k, fix_coord(y, T.o[1], ups_y), fix_coord(z, T.o[2], ups_z), (float)h, (float)m, (float)vx,
+                      (float)vy, (float)vz, (float)rho, (float)p2, (float)cs);
                 out_of_range = out_of_range || !fix_range_ok(y, z);
+                // bit 0: frozen (an FP64 pair); bit 1: in an FP64-band chunk
+                L.ind[k] = (unsigned char)((ind != 0 ? 1 : 0) | (pband != nullptr && pband[src] != 0 ? 2 : 0));
             } else {
                 L.y[k] = y;
                 L.z[k] = z;
                 L.h[k] = h;
-            }
-            if (KIND == 0) {
-                L.hm[k] = {(R)(0.5 * h), (R)(m * kInvPi)};
-            } else {
-                L.m[k] = (R)m;
-                L.vx[k] = (R)vx;
-                L.vy[k] = (R)vy;
-                L.vz[k] = (R)vz;
-                L.rho[k] = (R)rho;
-                L.pr2[k] = (R)p2;
-                L.cs[k] = (R)cs;
-                // bit 0: frozen (an FP64 pair); bit 1: in an FP64-band chunk
-                L.ind[k] = (unsigned char)((ind != 0 ? 1 : 0) | (pband != nullptr && pband[src] != 0 ? 2 : 0));
+                if (KIND == 0) {
+                    L.hm[k] = {(R)(0.5 * h), (R)(m * kInvPi)};
+                } else {
+                    L.m[k] = (R)m;
+                    L.vx[k] = (R)vx;
+                    L.vy[k] = (R)vy;
+                    L.vz[k] = (R)vz;
+                    L.rho[k] = (R)rho;
+                    L.pr2[k] = (R)p2;
+                    L.cs[k] = (R)cs;
+                    // bit 0: frozen (an FP64 pair); bit 1: in an FP64-band chunk
+                    L.ind[k] = (unsigned char)((ind != 0 ? 1 : 0) | (pband != nullptr && pband[src] != 0 ? 2 : 0));
+                }
             }
             if (kStats) L.id[k] = id;
         }
@@ -1388,7 +1430,7 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                         tz = B.z[wo + qq];
                         th = B.h[wo + qq];
                     } else {
-                        c = P4{L.x[u], (double)L.y[u], (double)L.z[u], (double)L.h[u]};
+                        c = P4{L.x[u], (double)L.Y(u), (double)L.Z(u), (double)L.H(u)};
                     }
                     const uint32_t hit = g.shifted ? pred1_shift(c, xi, ty, tz, th, g.shy, g.shz)
                                                    : pred1(c, xi, ty, tz, th);
@@ -1450,22 +1492,22 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                 const float fx = (float)dx;
                 float fy, fz, dd;
                 if (!far_mirror) {
-                    fy = (float)fix_diff(fix_coord(L.y[u], T.o[1], ups_y), Yi, sy) * qy;
-                    fz = (float)fix_diff(fix_coord(L.z[u], T.o[2], ups_z), Zi, sz) * qz;
+                    fy = (float)fix_diff(fix_coord(L.Y(u), T.o[1], ups_y), Yi, sy) * qy;
+                    fz = (float)fix_diff(fix_coord(L.Z(u), T.o[2], ups_z), Zi, sz) * qz;
                     dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
                 } else {
-                    const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
-                    const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+                    const double dy = kSh ? (L.Y(u) - yi) + shy : L.Y(u) - yi;
+                    const double dz = kSh ? (L.Z(u) - zi) + shz : L.Z(u) - zi;
                     fy = (float)dy;
                     fz = (float)dz;
                     dd = (float)__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
                 }
                 if (dd > 0.0f) {
                     float coef, edc;
-                    force_terms<float>(dd, 0.5f * (hif + (float)L.h[u]), fx, fy, fz, (float)vxi - (float)L.vx[u],
-                                       (float)vyi - (float)L.vy[u], (float)vzi - (float)L.vz[u], (float)rhoi_d,
-                                       (float)L.rho[u], (float)pr2i_d, (float)L.pr2[u], (float)csi_d, (float)L.cs[u],
-                                       (float)mi_d, (float)L.m[u], (float)T.alpha, (float)T.beta, (float)T.eps, plain,
+                    force_terms<float>(dd, 0.5f * (hif + (float)L.H(u)), fx, fy, fz, (float)vxi - (float)L.VX(u),
+                                       (float)vyi - (float)L.VY(u), (float)vzi - (float)L.VZ(u), (float)rhoi_d,
+                                       (float)L.RHO(u), (float)pr2i_d, (float)L.PR2(u), (float)csi_d, (float)L.CS(u),
+                                       (float)mi_d, (float)L.M(u), (float)T.alpha, (float)T.beta, (float)T.eps, plain,
                                        coef, edc);
                     a0.add(fx * coef);
                     a1.add(fy * coef);
@@ -1503,8 +1545,8 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                 const float fx = (float)dx;
                 float fy, fz, dd;
                 if (!far_piece) {
-                    fy = (float)fix_diff(L.y[u], Yi, sy) * qy;
-                    fz = (float)fix_diff(L.z[u], Zi, sz) * qz;
+                    fy = (float)fix_diff(L.Y(u), Yi, sy) * qy;
+                    fz = (float)fix_diff(L.Z(u), Zi, sz) * qz;
                     dd = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
                 } else {
                     int qq = q;
@@ -1518,8 +1560,8 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
                 }
                 if (dd > 0.0f) {
                     R coef, edc;
-                    force_terms<R>(dd, 0.5f * (hif + L.h[u]), fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
-                                   vzi_r - L.vz[u], rhoi, L.rho[u], pr2i, L.pr2[u], csi, L.cs[u], mi, L.m[u],
+                    force_terms<R>(dd, 0.5f * (hif + L.H(u)), fx, fy, fz, vxi_r - L.VX(u), vyi_r - L.VY(u),
+                                   vzi_r - L.VZ(u), rhoi, L.RHO(u), pr2i, L.PR2(u), csi, L.CS(u), mi, L.M(u),
                                    alpha, beta, epsv, plain, coef, edc);
                     a0.add(fx * coef);
                     a1.add(fy * coef);
@@ -1532,8 +1574,8 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
         if constexpr (KIND == 0) {
             // every LDS read of the hit is issued before the arithmetic waits on any
             const auto hm = L.hm[u];
-            const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
-            const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+            const double dy = kSh ? (L.Y(u) - yi) + shy : L.Y(u) - yi;
+            const double dz = kSh ? (L.Z(u) - zi) + shz : L.Z(u) - zi;
             const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
             const R r = sqrt_r<R>((R)dd);
             const R ih = rcp<R>(hhi + hm.hh);
@@ -1544,11 +1586,11 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             }
             return;
         }
-        const double dy = kSh ? (L.y[u] - yi) + shy : L.y[u] - yi;
-        const double dz = kSh ? (L.z[u] - zi) + shz : L.z[u] - zi;
+        const double dy = kSh ? (L.Y(u) - yi) + shy : L.Y(u) - yi;
+        const double dz = kSh ? (L.Z(u) - zi) + shz : L.Z(u) - zi;
         const double dd = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
         if (dd > 0.0) {
-            const double hhd = 0.5 * (hi + L.h[u]);
+            const double hhd = 0.5 * (hi + L.H(u));
             if (sizeof(R) == 4 && (L.ind[u] & 1) != 0) {
                 // a frozen neighbour has no partner term: FP64 keeps the momentum it
                 // injects exact to rounding
@@ -1568,8 +1610,8 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
             } else {
                 R coef, edc;
                 const R fx = (R)dx, fy = (R)dy, fz = (R)dz;
-                force_terms<R>((R)dd, (R)hhd, fx, fy, fz, vxi_r - L.vx[u], vyi_r - L.vy[u],
-                               vzi_r - L.vz[u], rhoi, L.rho[u], pr2i, L.pr2[u], csi, L.cs[u], mi, L.m[u], alpha,
+                force_terms<R>((R)dd, (R)hhd, fx, fy, fz, vxi_r - L.VX(u), vyi_r - L.VY(u),
+                               vzi_r - L.VZ(u), rhoi, L.RHO(u), pr2i, L.PR2(u), csi, L.CS(u), mi, L.M(u), alpha,
                                beta, epsv, plain, coef, edc);
                 a0.add(fx * coef);
                 a1.add(fy * coef);
