@@ -1173,8 +1173,8 @@ struct TileLds<R, KIND, kStats, kTileP, true> {
     I2 yz[kTileP];    // y, z (fixed point)
     F2 hm[kTileP];    // h, m
     F2 vxy[kTileP];   // vx, vy
-    F2 vzr[kTileP];   // vz, rho
-    F2 pc[kTileP];    // pr2, cs
+    F2 vzp[kTileP];   // vz, pr2
+    F2 rc[kTileP];    // rho, cs (read by the viscosity term only)
     unsigned char ind[kTileP];
     int id[kStats ? kTileP : 1];
     __device__ int Y(int u) const { return yz[u].a; }
@@ -1183,17 +1183,17 @@ struct TileLds<R, KIND, kStats, kTileP, true> {
     __device__ float M(int u) const { return hm[u].b; }
     __device__ float VX(int u) const { return vxy[u].a; }
     __device__ float VY(int u) const { return vxy[u].b; }
-    __device__ float VZ(int u) const { return vzr[u].a; }
-    __device__ float RHO(int u) const { return vzr[u].b; }
-    __device__ float PR2(int u) const { return pc[u].a; }
-    __device__ float CS(int u) const { return pc[u].b; }
+    __device__ float VZ(int u) const { return vzp[u].a; }
+    __device__ float PR2(int u) const { return vzp[u].b; }
+    __device__ float RHO(int u) const { return rc[u].a; }
+    __device__ float CS(int u) const { return rc[u].b; }
     __device__ void set(int k, int y, int z, float h, float m, float vx, float vy, float vz, float rho, float pr2,
                         float cs) {
         yz[k] = I2{y, z};
         hm[k] = F2{h, m};
         vxy[k] = F2{vx, vy};
-        vzr[k] = F2{vz, rho};
-        pc[k] = F2{pr2, cs};
+        vzp[k] = F2{vz, pr2};
+        rc[k] = F2{rho, cs};
     }
 };
 
