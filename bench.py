@@ -337,61 +337,85 @@ def run_alone(spec, variant=1):
 
 
 class SweepArgs:
-    """Defaults of the cfg3 sweep as the main benchmark line runs it."""
-    variant, tubes, chunk, serial_stride = 1, 512, 16, 8
+    """Defaults of the cfg3 sweep as the main benchmark line runs it; `split` batches: the
+    alternative policy reported beside it (the lighter half by estimated work in a batch of
+    its own, beside the rest)."""
+    variant, tubes, chunk, serial_stride, split = 1, 512, 16, 8, 2
 
 
-class ServeArgs:
-    """Defaults of the cfg4 stream as the main benchmark line runs it.  heavy: requests above
-    this many estimated particle-steps start one at a time (the large tubes, ~3.9e8; the sweep
-    tubes stay below 4e7)."""
-    variant, requests, load, sat_rate, budget, epoch, seed, heavy = 1, 60, 0.8, 0.0, 200e9, 8, 7, 1e8
+def _serial_model(sample):
+    """Least-squares fit of one-tube-at-a-time run times, seconds = steps x (a + b x particles),
+    to measured (particles, steps, seconds) samples (relative residuals); returns (a, b) and
+    the largest relative deviation of the fit from a sample."""
+    P = np.array([x[0] for x in sample], float)
+    S = np.array([x[1] for x in sample], float)
+    D = np.array([x[2] for x in sample], float)
+    A = np.stack([S / D, S * P / D], axis=1)
+    (a, b), *_ = np.linalg.lstsq(A, np.ones_like(D), rcond=None)
+    dev = float(np.max(np.abs(S * (a + b * P) / D - 1.0)))
+    return float(a), float(b), dev
 
 
 def sweep_result(args) -> dict:
-    """cfg3: 512 independent tubes in one batch, each run to t=0.2, against the same tubes
-    run one at a time (a sample of every `serial_stride`-th tube) on the same GPU."""
+    """cfg3: 512 independent tubes, each run to t=0.2, all submitted at once: in one batch, and
+    (reported beside it) lightest half first in a concurrent batch of its own, against the same
+    tubes run one at a time on the same GPU.  Every `serial_stride`-th tube is run alone and
+    timed; the serial queue's times to solution (one tube after another in submission order)
+    come from a per-step cost fitted to those runs."""
     from dustyshock3d_amd import configs
-    from dustyshock3d_amd.batch import Batch
+    from dustyshock3d_amd.batch import run_sweep, sweep_groups
+    from dustyshock3d_amd.sim import tube_count
 
     specs = configs.cfg3(args.tubes)
-    t_setup = time.perf_counter()
-    b = Batch(specs)
-    b.set_pair_variant(args.variant)
-    b.run(0)
-    setup = time.perf_counter() - t_setup
-    t0 = time.perf_counter()
-    finish = {}
-    while len(finish) < len(specs):
-        b.run(args.chunk)
-        now = time.perf_counter() - t0
-        for i in range(len(specs)):
-            if i not in finish and b.status(i)["done"]:
-                finish[i] = now
-    wall = time.perf_counter() - t0
-    ps = sum(2 * b.counts[i] * b.status(i)["steps"] for i in range(len(specs)))
-    steps = [b.status(i)["steps"] for i in range(len(specs))]
-    del b
-    sample = list(range(0, len(specs), args.serial_stride))
-    sps, sdt = 0.0, 0.0
-    for i in sample:
+    parts = [2 * tube_count(sp) for sp in specs]
+    runs = {}
+    for nb in (1, args.split):
+        t0 = time.perf_counter()
+        r = run_sweep(specs, batches=nb, chunk=args.chunk, pair_variant=args.variant, concurrent=True)
+        wall = time.perf_counter() - t0
+        ps = sum(p * st for p, st in zip(parts, r["steps"]))
+        tts = np.array(r["finish_s"])
+        runs[nb] = {"value": ps / wall, "wall_s": wall, "particle_steps": ps, "steps": r["steps"],
+                    "time_to_solution_s": {"p50": float(np.percentile(tts, 50)),
+                                           "p99": float(np.percentile(tts, 99))}}
+    steps = runs[1]["steps"]
+    if runs[args.split]["steps"] != steps:
+        raise RuntimeError("a tube took a different number of steps in a different batch")
+    sample_ids = list(range(0, len(specs), args.serial_stride))
+    sample, sps, sdt = [], 0.0, 0.0
+    for i in sample_ids:
         p, d = run_alone(specs[i], args.variant)
         sps += p
         sdt += d
+        sample.append((parts[i], p / parts[i], d))
     serial_rate = sps / sdt
-    tts = np.array(sorted(finish.values()))
+    a, b, dev = _serial_model(sample)
+    service = np.array([st * (a + b * p) for p, st in zip(parts, steps)])
+    serial_tts = np.cumsum(service)
+    head, alt = runs[1], runs[args.split]
     return {
         "metric": "sweep particle-steps/s (cfg3, 512 tubes in one batch)",
-        "value": ps / wall, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
-        "vs_serial_gpu": (ps / wall) / serial_rate,
+        "value": head["value"], "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "vs_serial_gpu": head["value"] / serial_rate,
         "serial_gpu_particle_steps_per_s": serial_rate,
-        "serial_sample": f"{len(sample)} of {len(specs)} tubes run one at a time, {sdt:.1f} s",
-        "wall_s": wall, "setup_s": setup, "particle_steps": ps,
+        "serial_sample": f"{len(sample_ids)} of {len(specs)} tubes run one at a time, {sdt:.1f} s",
+        "wall_s": head["wall_s"], "particle_steps": head["particle_steps"],
         "tubes": len(specs), "steps_min_max": [min(steps), max(steps)],
-        "time_to_solution_s": {"p50": float(np.percentile(tts, 50)), "p99": float(np.percentile(tts, 99))},
+        "time_to_solution_s": head["time_to_solution_s"],
+        "serial_fifo_time_to_solution_s": {"p50": float(np.percentile(serial_tts, 50)),
+                                           "p99": float(np.percentile(serial_tts, 99)),
+                                           "model": f"seconds = steps x ({a:.3e} + {b:.3e} x particles), "
+                                                    f"fitted to the {len(sample)} tubes run alone "
+                                                    f"(largest deviation {dev:.1%})"},
+        "lightest_half_first": {"batches": [len(g) for g in sweep_groups(specs, args.split)],
+                                "value": alt["value"], "wall_s": alt["wall_s"],
+                                "time_to_solution_s": alt["time_to_solution_s"],
+                                "policy": "tubes in ascending estimated work (particles x CFL steps at rest); "
+                                          "the lighter half in a batch of its own, run concurrently beside "
+                                          "the rest (own stream and host thread)"},
         "dtype": "f32 pair terms accumulated in f64 + f64 wall band, drag, positions, reductions", "data": "synthetic",
         "config": {"workload": "cfg3 sweep: y,z in [0,0.05], dx log-uniform [0.0025,0.01], K log-uniform "
-                               "[0.01,1000], dust-to-gas uniform [0.01,10], each to t=0.2",
+                               "[0.01,1000], dust-to-gas uniform [0.01,10], each to t=0.2, all submitted at t=0",
                    "pair_kernels": "tiled" if args.variant == 1 else "direct"},
     }
 
@@ -510,6 +534,7 @@ if __name__ == "__main__":
         ap.add_argument("--tubes", type=int, default=512)
         ap.add_argument("--chunk", type=int, default=16)
         ap.add_argument("--serial-stride", type=int, default=8)
+        ap.add_argument("--split", type=int, default=2, help="sweep: batches of the lightest-first policy")
         ap.add_argument("--requests", type=int, default=60)
         ap.add_argument("--load", type=float, default=0.8)
         ap.add_argument("--sat-rate", type=float, default=0.0)

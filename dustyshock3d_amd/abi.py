@@ -231,6 +231,7 @@ def _declare(lib):
         "dsh_server_result": (C.c_int, [vp, C.c_uint64, C.c_int, P(SoaView)]),
         "dsh_server_cancel": (C.c_int, [vp, C.c_uint64]),
         "dsh_server_footprint": (C.c_int, [P(TubeSpec), _DP]),
+        "dsh_server_work": (C.c_int, [P(TubeSpec), _DP]),
         "dsh_server_stats": (C.c_int, [vp, _DP, _IP, _IP]),
         "dsh_server_times": (C.c_int, [vp, _DP, _DP]),
         "dsh_server_last_error": (C.c_char_p, [vp]),

@@ -2,6 +2,8 @@
 from __future__ import annotations
 
 import ctypes as C
+import threading
+import time
 from typing import Callable, Optional, Sequence
 
 import numpy as np
@@ -114,6 +116,104 @@ def footprint(spec: TubeSpec) -> float:
     if rc != abi.OK:
         raise DshError(rc, "dsh_server_footprint")
     return b.value
+
+
+def work(spec: TubeSpec) -> float:
+    """Estimated work of a tube: particles x estimated steps (the server's cost estimate)."""
+    w = C.c_double()
+    rc = abi.lib().dsh_server_work(C.byref(spec), C.byref(w))
+    if rc != abi.OK:
+        raise DshError(rc, "dsh_server_work")
+    return w.value
+
+
+def sweep_groups(specs: Sequence[TubeSpec], batches: int = 2) -> list:
+    """Tube indices of a sweep's batches, lightest first.
+
+    The tubes are ordered by estimated work (ties by index); each batch but the last takes the
+    lighter half of the tubes not yet placed, the last takes the rest.  batches = 1: one batch
+    of all tubes in index order."""
+    n = len(specs)
+    if batches <= 1 or n < 2:
+        return [list(range(n))]
+    w = np.array([work(sp) for sp in specs])
+    order = [int(i) for i in np.argsort(w, kind="stable")]
+    groups = []
+    while len(groups) < batches - 1 and len(order) >= 2:
+        k = len(order) // 2
+        groups.append(order[:k])
+        order = order[k:]
+    groups.append(order)
+    return groups
+
+
+def run_sweep(specs: Sequence[TubeSpec], batches: int = 1, chunk: int = 16,
+              pair_variant: int = abi.PAIR_TILED, on_done: Optional[Callable] = None,
+              clock: Callable[[], float] = time.perf_counter, concurrent: bool = True) -> dict:
+    """Run every tube of a sweep to its end in the batches of sweep_groups.
+
+    concurrent: every batch has its own stream and host thread, so the batches share the GPU.
+    In one batch every tube advances one step per batch step, so a light tube's time to
+    solution is its steps times the step of the whole batch; in a batch of its own, beside the
+    heavy one, it runs at its own batch's pace.  Otherwise the batches run one after another.
+
+    Returns per tube (index order) the time to solution in seconds from the call, batch
+    set-ups included, and its steps.  on_done(i, batch, slot) is called once per tube when it
+    has finished, while its batch is alive (from that batch's thread when concurrent).  A
+    tube's results do not depend on the batch it runs in, nor on what runs beside it."""
+    t0 = clock()
+    n = len(specs)
+    finish, steps = [0.0] * n, [0] * n
+    groups = sweep_groups(specs, batches)
+
+    def make(g):
+        b = Batch([specs[i] for i in g])
+        b.set_pair_variant(pair_variant)
+        b.run(0)
+        return b
+
+    def drive(g, b):
+        left = list(range(len(g)))
+        while left:
+            b.run(chunk)
+            now = clock() - t0
+            still = []
+            for j in left:
+                st = b.status(j)
+                if st["done"]:
+                    finish[g[j]] = now
+                    steps[g[j]] = st["steps"]
+                    if on_done is not None:
+                        on_done(g[j], b, j)
+                else:
+                    still.append(j)
+            left = still
+
+    if concurrent and len(groups) > 1:
+        made = [make(g) for g in groups]
+        errors = []
+
+        def body(g, b):
+            try:
+                drive(g, b)
+            except BaseException as e:  # re-raised in the caller's thread
+                errors.append(e)
+
+        threads = [threading.Thread(target=body, args=(g, b)) for g, b in zip(groups, made)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for b in made:
+            b.close()
+        if errors:
+            raise errors[0]
+    else:
+        for g in groups:
+            b = make(g)
+            drive(g, b)
+            b.close()
+    return {"finish_s": finish, "steps": steps}
 
 
 class Server:

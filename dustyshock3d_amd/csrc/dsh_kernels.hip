@@ -788,6 +788,8 @@ __device__ __forceinline__ void force_terms(T d2, T hh, T fx, T fy, T fz, T dvx,
 // the tiled pair kernels): 1 if a particle of the chunk lies in its tube's FP64 band this
 // step.  Band chunks run the FP64 pair kernels; FP32 chunks evaluate their pairs with band
 // (and frozen) particles in FP64, so both particles of a pair compute its terms alike.
+// The chunk's class for the fused kernels' order: 1 band, 0 other, 2 a finished or failed
+// tube's (no pair work: left out of the order).
 __global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                               PBuf B, unsigned char* __restrict__ pband, const int* __restrict__ blk64,
                                               int* __restrict__ cflag) {
@@ -807,14 +809,17 @@ __global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict_
     // start each segment on a chunk boundary)
     if (cflag != nullptr && (threadIdx.x & 63) == 0) {
         const int ci = li >> 6;
-        if (ci < ((ip.sg.n + 63) >> 6)) cflag[blk64[find_seg(mp, blockIdx.x)] + ci] = any ? 1 : 0;
+        const int cls = (D.done || D.err) ? 2 : (any ? 1 : 0);
+        if (ci < ((ip.sg.n + 63) >> 6)) cflag[blk64[find_seg(mp, blockIdx.x)] + ci] = cls;
     }
 }
 
-// Chunk order of the fused mixed-precision pair kernel: the band's chunks ascending, then the
-// others ascending, in perm[0, nall), with the band count at perm[nall] and the count of band
-// chunks below gas_blocks (the gas chunks) at perm[nall + 1].  Two passes over blocks of
-// kPermThreads chunks: counts per block, then each block writes its chunks at its offsets.
+// Chunk order of the fused mixed-precision pair kernel in perm[0, nall): the band's chunks
+// ascending, then the other chunks of running tubes ascending, then the chunks of finished
+// tubes.  perm[nall] = band chunks, perm[nall + 1] = band chunks below gas_blocks (the gas
+// chunks), perm[nall + 2] = other running chunks, perm[nall + 3] = those below gas_blocks.
+// Two passes over blocks of kPermThreads chunks: counts per block (band, finished), then each
+// block writes its chunks at its offsets.
 constexpr int kPermThreads = 1024;
 __device__ __forceinline__ int block_sum_1024(int v, int* sh) {
     const int w = threadIdx.x >> 6;
@@ -833,39 +838,64 @@ __global__ __launch_bounds__(kPermThreads) void k_perm_count(const int* __restri
                                                             int* __restrict__ bcnt) {
     __shared__ int sh[kPermThreads / 64];
     const int b = blockIdx.x * kPermThreads + threadIdx.x;
-    const int c = block_sum_1024(b < nall ? cflag[b] : 0, sh);
-    if (threadIdx.x == 0) bcnt[blockIdx.x] = c;
-}
-__global__ __launch_bounds__(kPermThreads) void k_perm_write(const int* __restrict__ cflag, int nall,
-                                                            const int* __restrict__ bcnt, int gas_blocks,
-                                                            int* __restrict__ perm) {
-    __shared__ int sh[kPermThreads / 64];
-    __shared__ int wpre[kPermThreads / 64];
-    // band chunks of the blocks before this one, and of all blocks (nbk <= a few hundred)
-    int before = 0, total = 0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += kPermThreads) {
-        const int v = bcnt[k];
-        total += v;
-        before += k < (int)blockIdx.x ? v : 0;
-    }
-    before = block_sum_1024(before, sh);
-    total = block_sum_1024(total, sh);
-    const int b = blockIdx.x * kPermThreads + threadIdx.x;
     const int f = b < nall ? cflag[b] : 0;
-    // band chunks before b inside the block: wave prefix by ballot, then the waves before
-    const unsigned long long m = __ballot(f != 0);
+    const int band = block_sum_1024(f == 1 ? 1 : 0, sh);
+    const int gone = block_sum_1024(f == 2 ? 1 : 0, sh);
+    if (threadIdx.x == 0) {
+        bcnt[2 * blockIdx.x] = band;
+        bcnt[2 * blockIdx.x + 1] = gone;
+    }
+}
+// Count of the chunks of class c before this thread's chunk inside the block: wave prefix by
+// ballot, then the waves before it.
+__device__ __forceinline__ int class_before_in_block(bool is_c, int* wpre) {
+    const unsigned long long m = __ballot(is_c);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
     if (lane == 0) wpre[w] = __popcll(m);
     __syncthreads();
     int waves_before = 0;
     for (int k = 0; k < w; ++k) waves_before += wpre[k];
-    const int band_before = before + waves_before + in_wave;
+    __syncthreads();
+    return waves_before + in_wave;
+}
+__global__ __launch_bounds__(kPermThreads) void k_perm_write(const int* __restrict__ cflag, int nall,
+                                                            const int* __restrict__ bcnt, int gas_blocks,
+                                                            int* __restrict__ perm) {
+    __shared__ int sh[kPermThreads / 64];
+    __shared__ int wpre[kPermThreads / 64];
+    // band and finished chunks of the blocks before this one, and of all blocks (nbk <= a few
+    // hundred)
+    int bb = 0, bt = 0, gb = 0, gt = 0;
+    for (int k = threadIdx.x; k < (int)gridDim.x; k += kPermThreads) {
+        const int vb = bcnt[2 * k], vg = bcnt[2 * k + 1];
+        bt += vb;
+        gt += vg;
+        bb += k < (int)blockIdx.x ? vb : 0;
+        gb += k < (int)blockIdx.x ? vg : 0;
+    }
+    const int band_blocks_before = block_sum_1024(bb, sh);
+    const int band_total = block_sum_1024(bt, sh);
+    const int gone_blocks_before = block_sum_1024(gb, sh);
+    const int gone_total = block_sum_1024(gt, sh);
+    const int rest_total = nall - band_total - gone_total;
+    const int b = blockIdx.x * kPermThreads + threadIdx.x;
+    const int f = b < nall ? cflag[b] : 0;
+    const int band_before = band_blocks_before + class_before_in_block(b < nall && f == 1, wpre);
+    const int gone_before = gone_blocks_before + class_before_in_block(b < nall && f == 2, wpre);
+    const int rest_before = b - band_before - gone_before;
     if (b < nall) {
-        if (f) perm[band_before] = b;
-        else perm[total + (b - band_before)] = b;
-        if (b == nall - 1) perm[nall] = total;
-        if (b == gas_blocks - 1) perm[nall + 1] = band_before + f;
+        if (f == 1) perm[band_before] = b;
+        else if (f == 2) perm[band_total + rest_total + gone_before] = b;
+        else perm[band_total + rest_before] = b;
+        if (b == nall - 1) {
+            perm[nall] = band_total;
+            perm[nall + 2] = rest_total;
+        }
+        if (b == gas_blocks - 1) {
+            perm[nall + 1] = band_before + (f == 1 ? 1 : 0);
+            perm[nall + 3] = rest_before + (f == 0 ? 1 : 0);
+        }
     }
 }
 
@@ -1957,15 +1987,18 @@ constexpr int band_tile() {
 }
 
 // Mixed precision with an FP64 wall band, both paths in one launch: perm (built by
-// k_perm_write this step) lists the band's chunks first, then the others, each part in
-// ascending chunk order.  Blocks below the band count run the FP64 path on band chunks and
-// are dispatched first, so the band's slower waves overlap the FP32 ones instead of trailing
-// them; the rest run the FP32 path.  For the gas force map (KIND 1) the band part is the gas
-// band chunks (perm[nall + 1] of them) and the FP32 part starts at perm[nband_all].  The two
-// paths share one LDS tile per wave.  With kWaves waves per block, the band part takes
-// ceil(nband / kWaves) whole blocks (wave w of part block p: band chunk p * kWaves + w) and
-// the FP32 part the blocks after them; waves past a part's end leave at once.  The grid is
-// ceil(nchunks / kWaves) + 1 blocks, enough for both parts whatever the band count.
+// k_perm_write this step) lists the band's chunks first, then the other chunks of running
+// tubes, each part in ascending chunk order; finished tubes' chunks come last and get no
+// block.  Blocks below the band count run the FP64 path on band chunks and are dispatched
+// first, so the band's slower waves overlap the FP32 ones instead of trailing them; the next
+// blocks run the FP32 path.  For the gas force map (KIND 1) the parts are the gas chunks of
+// each (perm[nall + 1] and perm[nall + 3] of them) and the FP32 part starts at perm[nband_all].
+// The two paths share one LDS tile per wave.  With kWaves waves per block, the band part
+// takes ceil(nband / kWaves) whole blocks (wave w of part block p: band chunk p * kWaves + w)
+// and the FP32 part the ceil(nrest / kWaves) blocks after them, each part dealt to the XCDs in
+// contiguous stretches (so running chunks spread over every XCD however many tubes have
+// finished); waves past a part's end leave at once.  The grid, ceil(nchunks / kWaves) + 1
+// blocks, covers both parts whatever the counts; the blocks past them leave at once.
 template <int KIND, bool kStats, int kTileP, int kWaves>
 __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_fused(
     Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
@@ -1979,12 +2012,13 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIN
     constexpr size_t kTileBytes = ((sizeof(L32) > sizeof(L64) ? sizeof(L32) : sizeof(L64)) + 15) / 16 * 16;
     __shared__ __align__(16) unsigned char raw[kWaves][kTileBytes];
     const int wave = kWaves == 1 ? 0 : uni((int)(threadIdx.x >> 6));
-    const int nb = gridDim.x;
     const int band_all = uni(perm[nall]);
     const int nband = KIND == 0 ? band_all : uni(perm[nall + 1]);
-    const int nrest = mp.nblocks - nband;
+    const int nrest = KIND == 0 ? uni(perm[nall + 2]) : uni(perm[nall + 3]);
     const int bb = (nband + kWaves - 1) / kWaves;
+    const int br = (nrest + kWaves - 1) / kWaves;
     const int b = blockIdx.x;
+    if (b >= bb + br) return;   // blocks past both parts (finished tubes' chunks, padding)
     if (b < bb) {
         const int e = part_remap(b, 0, bb) * kWaves + wave;
         if (kWaves > 1 && e >= nband) return;
@@ -1992,7 +2026,7 @@ __global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIN
                                                              td, cur, B, start, wkey, reach, pr2, csav, ncount, nhash,
                                                              masks, mask_chunks, pband, frec);
     } else {
-        const int e = (part_remap(b, bb, nb) - bb) * kWaves + wave;
+        const int e = (part_remap(b, bb, bb + br) - bb) * kWaves + wave;
         if (e >= nrest) return;
         pairs_chunk<float, KIND, kStats, kTileP, kWaves>(*reinterpret_cast<L32*>(raw[wave]), uni(perm[band_all + e]),
                                                          mp, tc, td, cur, B, start, wkey, reach, pr2, csav, ncount,
@@ -2589,10 +2623,10 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     if (a.pband) {
         const int nall = a.pmap64.nblocks;
         hipLaunchKernelGGL(k_band, dim3(a.pmap.nblocks), b, 0, s, a.pmap, a.td, a.cur, a.B, a.pband, a.pmap64.blk,
-                           a.bperm ? a.bperm + nall + 2 : nullptr);
+                           a.bperm ? a.bperm + nall + 4 : nullptr);
         if (a.bperm && nall > 0) {
             const int nbk = (nall + kPermThreads - 1) / kPermThreads;
-            int* cflag = a.bperm + nall + 2;
+            int* cflag = a.bperm + nall + 4;
             int* bcnt = cflag + nall;
             hipLaunchKernelGGL(k_perm_count, dim3(nbk), dim3(kPermThreads), 0, s, cflag, nall, bcnt);
             hipLaunchKernelGGL(k_perm_write, dim3(nbk), dim3(kPermThreads), 0, s, cflag, nall, bcnt,

@@ -400,6 +400,14 @@ int dsh_server_footprint(const dsh_tube_spec* spec, double* bytes) {
     return DSH_OK;
 }
 
+int dsh_server_work(const dsh_tube_spec* spec, double* work) {
+    if (!spec || !work) return DSH_ERR_ARG;
+    int n[3];
+    if (dsh_tube_count(spec, n)) return DSH_ERR_ARG;
+    *work = (double)(n[0] + n[1]) * dsh::est_steps(*spec);
+    return DSH_OK;
+}
+
 int dsh_server_create(const dsh_server_opts* o, dsh_server** out) {
     if (!o || !out || o->hbm_budget <= 0) return DSH_ERR_ARG;
     std::unique_ptr<dsh::Backend> be;

@@ -307,6 +307,10 @@ int dsh_server_poll(dsh_server* sv, uint64_t id, int* state, double* arrival, do
 int dsh_server_result(dsh_server* sv, uint64_t id, int species, const dsh_soa_view* view);
 int dsh_server_cancel(dsh_server* sv, uint64_t id);
 int dsh_server_footprint(const dsh_tube_spec* spec, double* bytes);
+/* Estimated work of a request: particles x estimated steps (max_steps, or t_end over the CFL
+   step of the set-up at rest), the figure heavy_work is compared with.  A lower bound on the
+   steps (the CFL step shrinks as the flow speeds up) that orders requests by cost. */
+int dsh_server_work(const dsh_tube_spec* spec, double* work);
 int dsh_server_stats(dsh_server* sv, double* bytes_in_use, int* rebuilds, int* epochs);
 /* Host wall-clock seconds spent in batch rebuilds (allocation, adoption, set-up) and in
    advancing epochs (step launches and the wait for them), since creation. */

@@ -178,3 +178,33 @@ def test_graph_replays_match_eager_launches(precision):
             for k in FIELDS:
                 assert np.array_equal(dg[k], de[k]), (n, s, k)
         assert np.array_equal(g.conservation(), e.conservation())
+
+
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_sweep_lightest_half_first_matches_one_batch(concurrent):
+    """run_sweep in two batches (the lighter half by estimated work, and the rest), one after
+    the other or side by side on their own streams and threads, gives every tube the result
+    of one batch of all tubes, bit for bit."""
+    from dustyshock3d_amd.batch import run_sweep, sweep_groups
+
+    specs = small_specs(abi.MIXED)
+    print(f"\nsweep of {len(specs)} tubes in two batches, concurrent={concurrent}", flush=True)
+    got = {}
+
+    def keep(i, b, j):
+        got[i] = ([b.download(j, sp) for sp in (0, 1)], b.conservation(j))
+
+    r = run_sweep(specs, batches=2, chunk=5, on_done=keep, concurrent=concurrent)
+    assert sorted(got) == list(range(len(specs)))
+    light, heavy = sweep_groups(specs, 2)
+    if not concurrent:
+        assert max(r["finish_s"][i] for i in light) < min(r["finish_s"][i] for i in heavy)
+    ref = Batch(specs)
+    ref.run(400)
+    for i in range(len(specs)):
+        assert ref.status(i)["done"] and ref.status(i)["steps"] == r["steps"][i]
+        for sp in (0, 1):
+            rr = ref.download(i, sp)
+            for k in FIELDS:
+                assert np.array_equal(got[i][0][sp][k], rr[k]), (i, sp, k)
+        assert np.array_equal(got[i][1], ref.conservation(i))

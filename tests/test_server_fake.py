@@ -1,11 +1,15 @@
 """Scheduling logic of the continuous-batching server on the host-only fake device:
 arrival-order admission under the HBM budget, rejection, cancellation, recovery from a
 failed rebuild, finished tubes that wait for the next admission instead of forcing a
-rebuild, and the time accounting the p50/p99 report uses (injected clock)."""
+rebuild, and the time accounting the p50/p99 report uses (injected clock); the estimated
+work that orders a sweep's batches lightest first."""
+import math
+
 import pytest
 
 from dustyshock3d_amd import abi, configs
-from dustyshock3d_amd.batch import Server, footprint
+from dustyshock3d_amd.abi import DshError
+from dustyshock3d_amd.batch import Server, footprint, sweep_groups, work
 from dustyshock3d_amd.sim import tube_count
 
 
@@ -162,3 +166,33 @@ def test_heavy_requests_run_one_at_a_time_in_arrival_order():
     sv2 = Server(hbm_budget=1e12, steps_per_epoch=10, backend=abi.BACKEND_FAKE, clock=Clock())
     a, b = sv2.submit(heavy[0], 0.0), sv2.submit(heavy[1], 0.0)
     assert sv2.pump() == (2, 0)
+
+
+def test_sweep_groups_run_lightest_half_first():
+    # uneven tubes: work = particles x steps, the steps fixed by max_steps
+    specs = [configs.tube(w, dx, 1.0, 1.0, max_steps=st, tube_index=i)
+             for i, (w, dx, st) in enumerate([(0.05, 0.005, 40), (0.03, 0.01, 10), (0.04, 0.006, 25),
+                                              (0.03, 0.01, 30), (0.05, 0.008, 5), (0.02, 0.01, 20),
+                                              (0.04, 0.005, 10)])]
+    w = [work(sp) for sp in specs]
+    for sp, wk in zip(specs, w):
+        assert wk == 2 * tube_count(sp) * sp.max_steps
+    assert sweep_groups(specs, 1) == [list(range(len(specs)))]
+    g2 = sweep_groups(specs, 2)
+    assert [len(g) for g in g2] == [3, 4]
+    g3 = sweep_groups(specs, 3)
+    assert [len(g) for g in g3] == [3, 2, 2]
+    for groups in (g2, g3):
+        flat = [i for g in groups for i in g]
+        assert sorted(flat) == list(range(len(specs)))
+        # ascending work along the batches, ties in index order
+        assert [w[i] for i in flat] == sorted(w)
+        assert all(w[a] < w[b] or (w[a] == w[b] and a < b) for a, b in zip(flat, flat[1:]))
+
+
+def test_work_of_a_timed_tube_counts_cfl_steps_at_rest():
+    # t_end over the CFL step of the set-up at rest, cfl x 1.2 dx (unit sound speed)
+    sp = configs.tube(0.03, 0.01, 1.0, 1.0, t_end=0.2, dt_mode=abi.DT_CFL, cfl=0.3)
+    assert work(sp) == 2 * tube_count(sp) * math.ceil(0.2 / (0.3 * 1.2 * 0.01))
+    with pytest.raises(DshError):
+        work(configs.tube(0.03, -0.01, 1.0, 1.0, max_steps=5))
