@@ -343,6 +343,13 @@ class SweepArgs:
     variant, tubes, chunk, serial_stride, split = 1, 512, 16, 8, 2
 
 
+class ServeArgs:
+    """Defaults of the cfg4 stream as the main benchmark line runs it.  heavy: requests above
+    this many estimated particle-steps start one at a time (the large tubes, ~3.9e8; the sweep
+    tubes stay below 4e7)."""
+    variant, requests, load, sat_rate, budget, epoch, seed, heavy = 1, 60, 0.8, 0.0, 200e9, 8, 7, 1e8
+
+
 def _serial_model(sample):
     """Least-squares fit of one-tube-at-a-time run times, seconds = steps x (a + b x particles),
     to measured (particles, steps, seconds) samples (relative residuals); returns (a, b) and
