@@ -161,6 +161,9 @@ def run_sweep(specs: Sequence[TubeSpec], batches: int = 1, chunk: int = 16,
     set-ups included, and its steps.  on_done(i, batch, slot) is called once per tube when it
     has finished, while its batch is alive (from that batch's thread when concurrent).  A
     tube's results do not depend on the batch it runs in, nor on what runs beside it."""
+    for i, sp in enumerate(specs):
+        if not (sp.t_end < 1e29 or sp.max_steps > 0):
+            raise ValueError(f"tube {i} has no end (t_end or max_steps)")
     t0 = clock()
     n = len(specs)
     finish, steps = [0.0] * n, [0] * n

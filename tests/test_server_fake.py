@@ -9,7 +9,7 @@ import pytest
 
 from dustyshock3d_amd import abi, configs
 from dustyshock3d_amd.abi import DshError
-from dustyshock3d_amd.batch import Server, footprint, sweep_groups, work
+from dustyshock3d_amd.batch import Server, footprint, run_sweep, sweep_groups, work
 from dustyshock3d_amd.sim import tube_count
 
 
@@ -196,3 +196,9 @@ def test_work_of_a_timed_tube_counts_cfl_steps_at_rest():
     assert work(sp) == 2 * tube_count(sp) * math.ceil(0.2 / (0.3 * 1.2 * 0.01))
     with pytest.raises(DshError):
         work(configs.tube(0.03, -0.01, 1.0, 1.0, max_steps=5))
+
+
+def test_sweep_rejects_a_tube_without_an_end():
+    # checked before any batch is built, so no device is needed
+    with pytest.raises(ValueError):
+        run_sweep([configs.tube(0.03, 0.01, 1.0, 1.0, max_steps=5), configs.tube(0.03, 0.01, 1.0, 1.0)])
