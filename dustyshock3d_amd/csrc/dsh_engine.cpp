@@ -318,9 +318,12 @@ int Engine::prepare() {
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].precision == DSH_MIXED && tc_h_[t].wall_band) {
             if ((rc = alloc(&a_.pband, (size_t)ntot_))) return rc;
-            if ((rc = alloc(&a_.bperm, bperm_ints(a_.pmap64.nblocks)))) return rc;
             break;
         }
+    // the pair kernels' chunk order: band chunks first (fused kernels), finished tubes' chunks
+    // left out (a batch of several tubes)
+    if (a_.pband || T > 1)
+        if ((rc = alloc(&a_.bperm, bperm_ints(a_.pmap64.nblocks)))) return rc;
     a_.frec = nullptr;
     for (int t = 0; t < T; ++t)
         if (tc_h_[t].precision == DSH_MIXED) {

@@ -83,9 +83,10 @@ struct StepArgs {
     // mixed precision with an FP64 wall band: per sorted particle, 1 if its 64-particle chunk
     // has a particle in its tube's band (written by k_band each step); null otherwise
     unsigned char* pband;
-    // with pband: the chunk order of the fused pair kernels (pmap64.nblocks entries; then the
-    // band count, gas band count, count of the other running chunks and of their gas chunks),
-    // then per-chunk classes (pmap64.nblocks) and two counts per block of the ordering kernels
+    // with pband or several tubes (else null): the chunk order of the pair kernels
+    // (pmap64.nblocks entries; then the band count, gas band count, count of the other running
+    // chunks and of their gas chunks), then per-chunk classes (pmap64.nblocks) and two counts
+    // per block of the ordering kernels
     int* bperm;
 };
 // ints of StepArgs::bperm for nall pmap64 chunks

@@ -799,12 +799,12 @@ __global__ __launch_bounds__(kBS) void k_band(Map mp, const TubeDyn* __restrict_
     const TubeDyn& D = td[ip.sg.tube];
     const long long wo = (long long)(1 - *cur) * B.ntot;
     bool in = false;
-    if (valid) {
+    if (valid && pband != nullptr) {
         const double x = B.x[wo + ip.sg.off + li];
         in = x < D.band_lo || x > D.band_hi;
     }
     const bool any = __ballot(in) != 0ull;
-    if (valid) pband[ip.sg.off + li] = any ? 1 : 0;
+    if (valid && pband != nullptr) pband[ip.sg.off + li] = any ? 1 : 0;
     // the chunk's flag by its pmap64 block (pmap and pmap64 list the same segments, and both
     // start each segment on a chunk boundary)
     if (cflag != nullptr && (threadIdx.x & 63) == 0) {
@@ -1942,23 +1942,6 @@ __device__ __forceinline__ void pairs_chunk(TileLds<R, KIND, kStats, kTileP>& L,
     }
 }
 
-// Every chunk of the map, kWaves per block (wave w of block b takes chunk b * kWaves + w;
-// blocks remapped so each XCD takes a contiguous run).
-template <typename R, int KIND, bool kStats, int kTileP, int kWaves>
-__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(
-    Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
-    const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
-    double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
-    unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
-    const unsigned char* __restrict__ pband, ForceRec* __restrict__ frec) {
-    __shared__ TileLds<R, KIND, kStats, kTileP> L[kWaves];
-    const int wave = kWaves == 1 ? 0 : uni((int)(threadIdx.x >> 6));
-    const int b = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
-    if (kWaves > 1 && b >= mp.nblocks) return;
-    pairs_chunk<R, KIND, kStats, kTileP, kWaves>(L[wave], b, mp, tc, td, cur, B, start, wkey, reach, pr2, csav,
-                                                 ncount, nhash, masks, mask_chunks, pband, frec);
-}
-
 // Block b of the part [p0, p1) of a grid, remapped so that each XCD (which takes the blocks
 // b with b % 8 equal to its index) runs one contiguous stretch of the part.
 __device__ __forceinline__ int part_remap(int b, int p0, int p1) {
@@ -1972,6 +1955,35 @@ __device__ __forceinline__ int part_remap(int b, int p0, int p1) {
     }
     const int first = p0 + ((x - p0) & 7);
     return p0 + before + ((b - first) >> 3);
+}
+
+// Every chunk of the map, kWaves per block (wave w of block b takes chunk b * kWaves + w;
+// blocks remapped so each XCD takes a contiguous run).  With perm (a batch of several tubes):
+// the running tubes' chunks only, perm[0, perm[nall + 2]) (gas map: perm[nall + 3]), dealt to
+// the XCDs the same way, so they spread over every XCD however many tubes have finished.
+template <typename R, int KIND, bool kStats, int kTileP, int kWaves>
+__global__ __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(KIND == 0 ? 5 : 4))) void k_pairs_tiled(
+    Map mp, const TubeConst* __restrict__ tc, const TubeDyn* __restrict__ td, const int* __restrict__ cur, PBuf B,
+    const int* __restrict__ start, const int* __restrict__ wkey, const double* __restrict__ reach,
+    double* __restrict__ pr2, double* __restrict__ csav, int* __restrict__ ncount,
+    unsigned long long* __restrict__ nhash, unsigned long long* __restrict__ masks, int mask_chunks,
+    const unsigned char* __restrict__ pband, ForceRec* __restrict__ frec, const int* __restrict__ perm, int nall) {
+    __shared__ TileLds<R, KIND, kStats, kTileP> L[kWaves];
+    const int wave = kWaves == 1 ? 0 : uni((int)(threadIdx.x >> 6));
+    int b;
+    if (perm != nullptr) {
+        const int nrun = KIND == 0 ? uni(perm[nall + 2]) : uni(perm[nall + 3]);
+        const int br = (nrun + kWaves - 1) / kWaves;
+        if ((int)blockIdx.x >= br) return;
+        const int e = part_remap(blockIdx.x, 0, br) * kWaves + wave;
+        if (e >= nrun) return;
+        b = uni(perm[e]);
+    } else {
+        b = xcd_remap(blockIdx.x, gridDim.x) * kWaves + wave;
+        if (kWaves > 1 && b >= mp.nblocks) return;
+    }
+    pairs_chunk<R, KIND, kStats, kTileP, kWaves>(L[wave], b, mp, tc, td, cur, B, start, wkey, reach, pr2, csav,
+                                                 ncount, nhash, masks, mask_chunks, pband, frec);
 }
 
 // Pieces of the FP64 wall-band path inside the fused kernels: the largest of 128, 96, 64
@@ -2618,9 +2630,9 @@ hipError_t launch_radius(const StepArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp, 0);
     hipLaunchKernelGGL(k_window_max, g, b, 0, s, a.cmap, a.tc, a.td, a.wtmp, a.wtmp + a.ncells, 1);
     hipLaunchKernelGGL(k_radius, g, b, 0, s, a.cmap, a.tc, a.td, a.cellh, a.wtmp + a.ncells, a.reach);
-    // chunk flags of the FP64 wall band (mixed precision) and the chunk order of the fused
-    // pair kernels (band chunks first), read by both pair passes
-    if (a.pband) {
+    // chunk classes (FP64 wall band in mixed precision, finished tubes) and the chunk order of
+    // the tiled and fused pair kernels (band chunks first), read by both pair passes
+    if (a.pband || a.bperm) {
         const int nall = a.pmap64.nblocks;
         hipLaunchKernelGGL(k_band, dim3(a.pmap.nblocks), b, 0, s, a.pmap, a.td, a.cur, a.B, a.pband, a.pmap64.blk,
                            a.bperm ? a.bperm + nall + 4 : nullptr);
@@ -2641,7 +2653,7 @@ static void launch_tiled_t(const StepArgs& a, const Map& m, const unsigned char*
     const int grid = (m.nblocks + kWaves - 1) / kWaves;
     hipLaunchKernelGGL((k_pairs_tiled<R, KIND, kStats, kTileP, kWaves>), dim3(grid), dim3(64 * kWaves), 0, s, m, a.tc,
                        a.td, a.cur, a.B, a.start, a.wkey, a.reach, a.pr2, a.csav, a.ncount, a.nhash, a.masks,
-                       a.gmap64.nblocks, pband, a.frec);
+                       a.gmap64.nblocks, pband, a.frec, a.bperm, a.pmap64.nblocks);
 }
 
 template <int KIND, bool kStats, int kTileP, int kWaves>
