@@ -103,7 +103,9 @@ int Engine::device_fields(int tube, int species, int* cur_out, long long* off_ou
     if (!prepared_ || tube < 0 || tube >= ntubes() || species < 0 || species >= tc_h_[tube].nsp) return DSH_ERR_ARG;
     int cur = 0;
     if (hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return DSH_ERR_HIP;
-    *cur_out = cur;
+    TubeDyn d;
+    if (hipMemcpy(&d, a_.td + tube, sizeof(TubeDyn), hipMemcpyDeviceToHost) != hipSuccess) return DSH_ERR_HIP;
+    *cur_out = d.done ? d.done_cur : cur;   // a finished tube's state stays in done_cur
     *off_out = tc_h_[tube].p_off[species];
     return DSH_OK;
 }
@@ -397,6 +399,10 @@ int Engine::prepare() {
         if ((rc = hip_check(hipStreamSynchronize(S.stream_), "source sync"))) return rc;
         int scur = 0;
         if ((rc = hip_check(hipMemcpy(&scur, S.a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
+        TubeDyn sdyn;
+        if ((rc = hip_check(hipMemcpy(&sdyn, S.a_.td + st, sizeof(TubeDyn), hipMemcpyDeviceToHost), "D2H dyn")))
+            return rc;
+        if (sdyn.done) scur = sdyn.done_cur;   // a finished tube's state stays in done_cur
         for (int s = 0; s < tc_h_[t].nsp; ++s) {
             const size_t n = (size_t)tc_h_[t].np[s];
             const size_t so = (size_t)scur * S.ntot_ + (size_t)S.tc_h_[st].p_off[s];
@@ -707,8 +713,9 @@ int Engine::download(int tube, int species, const dsh_soa_view* v) {
     if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
     const TubeConst& C = tc_h_[tube];
     const TubeDyn& D = dyn_h_[tube];
-    const int derived = cur;   // finished tubes carry their last A along when re-sorted
-    (void)D;
+    // a finished tube is not re-sorted: its final state stays in buffer done_cur
+    if (D.done) cur = D.done_cur;
+    const int derived = cur;
     const size_t n = (size_t)C.np[species];
     const size_t off = (size_t)C.p_off[species];
     std::vector<int> ids(n), idd(n);
@@ -785,7 +792,10 @@ int Engine::sorted_ids(int tube, int species, int* ids) {
     if ((rc = hip_check(hipStreamSynchronize(stream_), "sorted ids sync"))) return rc;
     int cur = 0;
     if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
-    // the current buffer holds the last step's particles in that step's sorted order
+    if ((rc = refresh_dyn())) return rc;
+    // the current buffer holds the last step's particles in that step's sorted order (a
+    // finished tube's, in its last step's order, buffer done_cur)
+    if (dyn_h_[tube].done) cur = dyn_h_[tube].done_cur;
     return hip_check(hipMemcpy(ids, a_.B.id + (size_t)cur * ntot_ + tc_h_[tube].p_off[species],
                                sizeof(int) * (size_t)tc_h_[tube].np[species], hipMemcpyDeviceToHost),
                      "D2H sorted ids");
@@ -800,8 +810,7 @@ int Engine::neighbour_stats(int tube, int species, int* count, uint64_t* hash) {
     int cur = 0;
     if ((rc = hip_check(hipMemcpy(&cur, a_.cur, sizeof(int), hipMemcpyDeviceToHost), "D2H cur"))) return rc;
     const TubeDyn& D = dyn_h_[tube];
-    const int derived = cur;   // finished tubes carry their last A along when re-sorted
-    (void)D;
+    const int derived = D.done ? D.done_cur : cur;   // a finished tube's state stays in done_cur
     const size_t n = (size_t)tc_h_[tube].np[species];
     const size_t off = (size_t)tc_h_[tube].p_off[species];
     std::vector<int> ids(n), c(n);

@@ -565,6 +565,9 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
     __shared__ unsigned long long sx[kRankRun];
     __shared__ int sid[kRankRun];
     const ItemPos ip = locate(mp, kBS);
+    // a finished (or failed) tube is sorted no more: its final state stays in buffer done_cur,
+    // in the order of its last step (wkey), where downloads read it
+    if (td[ip.sg.tube].done || td[ip.sg.tube].err) return;
     const int li = ip.li + threadIdx.x;
     const int qf = ip.sg.off + ip.li;
     const int ql = ip.sg.off + min(ip.li + kBS, ip.sg.n) - 1;
@@ -604,23 +607,19 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
     const long long gs = so + src, gd = wo + dst;
     // every field is read before the first store: a store could alias any field as far as
     // the compiler knows, so loads left after stores would each wait for a round trip.
-    // Density, pressure and stopping time are recomputed this step for active particles of
-    // running tubes; frozen particles and finished tubes carry theirs along.
+    // Density, pressure and stopping time are recomputed this step for active particles;
+    // frozen particles carry theirs along.
     const bool gas = ip.sg.species == 0;
-    const bool done = td[ip.sg.tube].done != 0;
-    const bool tube_keep = done || td[ip.sg.tube].err;
     const double x = B.x[gs], y = B.y[gs], z = B.z[gs];
     const double vx = B.vx[gs], vy = B.vy[gs], vz = B.vz[gs];
     const double m = B.m[gs];
     hq = B.h[gs];
     const int ind = B.ind[gs];
-    const bool keep = ind != 0 || tube_keep;
+    const bool keep = ind != 0;
     const double rho = keep ? B.rho[gs] : 0.0;
     const double e = gas ? B.e[gs] : 0.0;
     const double pr = gas && keep ? B.p[gs] : 0.0;
     const double ts = !gas && keep ? B.ts[gs] : 0.0;
-    const double ax = gas && done ? B.ax[gs] : 0.0, ay = gas && done ? B.ay[gs] : 0.0;
-    const double az = gas && done ? B.az[gs] : 0.0, ed = gas && done ? B.ed[gs] : 0.0;
     B.x[gd] = x;
     B.y[gd] = y;
     B.z[gd] = z;
@@ -636,12 +635,6 @@ __global__ __launch_bounds__(kBS) void k_rank_permute(Map mp, TubeDyn* __restric
         if (ginv) ginv[ip.sg.off + my] = dst;
         B.e[gd] = e;
         if (keep) B.p[gd] = pr;
-        if (done) {
-            B.ax[gd] = ax;
-            B.ay[gd] = ay;
-            B.az[gd] = az;
-            B.ed[gd] = ed;
-        }
     } else if (keep) {
         B.ts[gd] = ts;
     }
@@ -695,13 +688,16 @@ __device__ __forceinline__ double axis_window_max(const TubeConst& T, const int 
     return m;
 }
 
-// The x and y passes of the separable window max (the z pass is taken inside k_radius).
+// The x and y passes of the separable window max (the z pass is taken inside k_radius).  A
+// finished tube's cells are skipped: it is not re-sorted, so its cell maxima stay zero and
+// k_radius gives its cells no reach without reading the window.
 __global__ __launch_bounds__(kBS) void k_window_max(Map mp, const TubeConst* __restrict__ tc,
                                                     const TubeDyn* __restrict__ td, const double* __restrict__ in,
                                                     double* __restrict__ out, int axis) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
+    if (td[ip.sg.tube].done || td[ip.sg.tube].err) return;
     const TubeConst& T = tc[ip.sg.tube];
     if (T.h_mode == 0) return;
     int w[3];

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_unpermute(PBuf B, const int* __restrict
                                                    dsh_soa_view out) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
-    const long long g = (long long)(*cur) * B.ntot + off + q;
+    // a tube that finished before the last step requested keeps its state in buffer done_cur
+    const long long g = (long long)(td->done ? td->done_cur : *cur) * B.ntot + off + q;
     const int id = B.id[g];
     struct {
         double* o;
