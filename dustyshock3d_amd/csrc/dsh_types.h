@@ -53,7 +53,7 @@ struct TubeConst {
 struct TubeDyn {
     double t, dt;
     int steps, done, final_step, err;
-    int done_cur;                  // buffer holding the derived fields of the final step
+    int done_cur;                  // buffer holding a finished tube's final state (not re-sorted)
     int pad_;
     unsigned long long hmax_bits[kMaxSpecies];  // max h per species this step (positive doubles)
     double cons[4];                // momentum (3), energy after the last step
