@@ -393,8 +393,12 @@ __global__ __launch_bounds__(kBS) void k_prepare(Map mp, const TubeConst* __rest
 
 // Cell key and slot.  Consecutive lanes hold consecutive particles of the previous
 // step's sorted order, so equal keys come in runs: one atomic per run, found with a
-// wave ballot of run heads.
-__global__ __launch_bounds__(kBS) void k_key(Map mp, const TubeConst* __restrict__ tc, const int* __restrict__ cur,
+// wave ballot of run heads.  A finished (or failed) tube is not re-sorted: its particles
+// are counted in its first cell of the species, which keeps every later tube's cell starts
+// (one prefix sum over all cells) in place, and get no key or slot (k_scatter and
+// k_rank_permute skip the tube).
+__global__ __launch_bounds__(kBS) void k_key(Map mp, const TubeConst* __restrict__ tc,
+                                             const TubeDyn* __restrict__ td, const int* __restrict__ cur,
                                              PBuf B, int* __restrict__ key, int* __restrict__ slot,
                                              int* __restrict__ count) {
     const ItemPos ip = locate(mp, kBS);
@@ -402,6 +406,10 @@ __global__ __launch_bounds__(kBS) void k_key(Map mp, const TubeConst* __restrict
     const bool valid = li < ip.sg.n;
     const int p = ip.sg.off + li;
     const TubeConst& T = tc[ip.sg.tube];
+    if (td[ip.sg.tube].done || td[ip.sg.tube].err) {
+        if (threadIdx.x == 0) atomicAdd(&count[T.cell_off[ip.sg.species]], min(kBS, ip.sg.n - ip.li));
+        return;
+    }
     const long long so = (long long)(*cur) * B.ntot;
     int k = -1;
     if (valid) {
@@ -505,12 +513,13 @@ __device__ __forceinline__ unsigned long long order_bits(double v) {
     return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
 }
 
-__global__ __launch_bounds__(kBS) void k_scatter(Map mp, const int* __restrict__ key, const int* __restrict__ slot,
-                                                 const int* __restrict__ start, int* __restrict__ tsrc,
-                                                 int* __restrict__ tkey) {
+__global__ __launch_bounds__(kBS) void k_scatter(Map mp, const TubeDyn* __restrict__ td, const int* __restrict__ key,
+                                                 const int* __restrict__ slot, const int* __restrict__ start,
+                                                 int* __restrict__ tsrc, int* __restrict__ tkey) {
     const ItemPos ip = locate(mp, kBS);
     const int li = ip.li + threadIdx.x;
     if (li >= ip.sg.n) return;
+    if (td[ip.sg.tube].done || td[ip.sg.tube].err) return;   // not re-sorted (k_key)
     const int p = ip.sg.off + li;
     const int k = key[p];
     const int pos = start[k] + slot[p];
@@ -2597,7 +2606,7 @@ hipError_t launch_prepare(const StepArgs& a, double* scratch, hipStream_t s) {
 }
 
 hipError_t launch_key(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_key, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.cur, a.B, a.key, a.slot,
+    hipLaunchKernelGGL(k_key, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.tc, a.td, a.cur, a.B, a.key, a.slot,
                        a.count);
     return hipGetLastError();
 }
@@ -2610,8 +2619,8 @@ hipError_t launch_scan(const StepArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_scatter(const StepArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.key, a.slot, a.start, a.tsrc,
-                       a.tkey);
+    hipLaunchKernelGGL(k_scatter, dim3(a.pmap.nblocks), dim3(kBS), 0, s, a.pmap, a.td, a.key, a.slot, a.start,
+                       a.tsrc, a.tkey);
     return hipGetLastError();
 }
 
